@@ -1,0 +1,330 @@
+"""Python binding of libbsar.so, the MI355X-native bistatic-SAR imaging path.
+
+Thin ctypes layer over the C ABI in include/bsar.h: numpy arrays in, numpy
+arrays out.  All compute runs in the HIP kernels of libbsar.so; there is no
+CPU fallback -- importing raises when the library has not been built.
+
+The directory name is not a Python identifier; load it with
+``load_package()`` from ``__graft_entry__`` or ``importlib`` by path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libbsar.so"
+
+FP64 = 0
+MIXED = 1
+
+_STATUS = {
+    0: "ok", -1: "invalid argument", -2: "device error", -3: "out of memory", -4: "i/o error",
+    -5: "format error", -6: "capacity exceeded", -7: "invalid state",
+}
+
+
+class BsarError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{_STATUS.get(status, status)} ({status}): {msg}")
+        self.status = status
+
+
+_dp = C.POINTER(C.c_double)
+
+
+class DemC(C.Structure):
+    _fields_ = [("x", _dp), ("nx", C.c_int), ("y", _dp), ("ny", C.c_int), ("z", _dp),
+                ("zrows", C.c_int), ("zcols", C.c_int)]
+
+
+class ParamsC(C.Structure):
+    _fields_ = [("fc", C.c_double), ("c", C.c_double), ("fs", C.c_double), ("n1", C.c_double),
+                ("n2", C.c_double), ("upsample", C.c_int), ("reserved", C.c_int)]
+
+
+class ReportC(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("voxel_pulses", C.c_double), ("voxels", C.c_int64),
+                ("pulses", C.c_int32), ("launches", C.c_int32), ("r1_exact_frac", C.c_double)]
+
+
+# name -> (restype, argtypes); every function declared in include/bsar.h
+SIGNATURES = {
+    "bsar_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "bsar_ctx_destroy": (None, [C.c_void_p]),
+    "bsar_last_error": (C.c_char_p, [C.c_void_p]),
+    "bsar_status_string": (C.c_char_p, [C.c_int]),
+    "bsar_abi_version": (C.c_int, []),
+    "bsar_ctx_synchronize": (C.c_int, [C.c_void_p]),
+    "bsar_dem_interp": (C.c_int, [C.c_void_p, C.POINTER(DemC), _dp, C.c_int64, _dp]),
+    "bsar_refraction_points": (C.c_int, [C.c_void_p, C.POINTER(DemC), _dp, C.c_int64, C.c_double, C.c_double, _dp]),
+    "bsar_slant_range_fermat": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int64, _dp, _dp]),
+    "bsar_upsample": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp]),
+    "bsar_range_compress": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp]),
+    "bsar_chirp": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
+    "bsar_dataset_create": (C.c_int, [C.c_void_p, C.POINTER(ParamsC), C.POINTER(DemC), _dp, _dp, C.c_int,
+                                      C.POINTER(C.c_void_p)]),
+    "bsar_dataset_destroy": (None, [C.c_void_p]),
+    "bsar_dataset_load_rootdata": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int, C.c_int]),
+    "bsar_dataset_load_upsampled": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int, C.c_int]),
+    "bsar_dataset_device_bytes": (C.c_int64, [C.c_void_p]),
+    "bsar_backproject": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp,
+                                   C.POINTER(ReportC)]),
+    "bsar_backproject_device": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int,
+                                          C.c_void_p, C.POINTER(ReportC)]),
+    "bsar_read_matrix": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.c_int, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int), _dp, C.POINTER(C.c_uint64)]),
+    "bsar_write_image": (C.c_int, [C.c_char_p, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "bsar_write_matrix2d": (C.c_int, [C.c_char_p, _dp, C.c_uint64, C.c_uint64]),
+    "bsar_flops_per_voxel_pulse": (C.c_double, [C.c_int]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libbsar.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        path = Path(os.environ.get("BSAR_LIB", LIB_PATH))
+        if not path.exists():
+            raise ImportError(f"{path} is missing: build it with `python c-_bistatic_sar_amd/build.py`")
+        L = C.CDLL(str(path))
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+@dataclass
+class Dem:
+    """ProcessedDEM: x[nx], y[ny] ascending, z[ny][nx] (row = y)."""
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+
+    def c(self):
+        self._x, self._y, self._z = _f64(self.x), _f64(self.y), _f64(self.z)
+        zr, zc = (self._z.shape if self._z.ndim == 2 else (0, 0))
+        s = DemC(_ptr(self._x), len(self._x), _ptr(self._y), len(self._y), _ptr(self._z), zr, zc)
+        return s
+
+
+@dataclass
+class Params:
+    fc: float
+    c: float
+    fs: float
+    n1: float
+    n2: float
+    upsample: int = 4
+
+    def c_struct(self):
+        return ParamsC(self.fc, self.c, self.fs, self.n1, self.n2, self.upsample, 0)
+
+
+class Context:
+    """One HIP stream on one GPU (bsar_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        st = self.L.bsar_ctx_create(device, C.byref(h))
+        if st != 0:
+            raise BsarError(st, f"bsar_ctx_create(device={device})")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.bsar_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, st: int, what: str):
+        if st != 0:
+            raise BsarError(st, f"{what}: {self.L.bsar_last_error(self.h).decode()}")
+
+    # -- stand-alone kernels ------------------------------------------------
+    def dem_interp(self, dem: Dem, xy) -> np.ndarray:
+        xy = _f64(xy).reshape(-1, 2)
+        out = np.empty(len(xy))
+        d = dem.c()
+        self._chk(self.L.bsar_dem_interp(self.h, C.byref(d), _ptr(xy), len(xy), _ptr(out)), "dem_interp")
+        return out
+
+    def refraction_points(self, dem: Dem, legs, n1: float, n2: float) -> np.ndarray:
+        legs = _f64(legs).reshape(-1, 6)
+        out = np.empty((len(legs), 3))
+        d = dem.c()
+        self._chk(self.L.bsar_refraction_points(self.h, C.byref(d), _ptr(legs), len(legs), n1, n2, _ptr(out)),
+                  "refraction_points")
+        return out
+
+    def upsample(self, data, factor: int = 4) -> np.ndarray:
+        """data [N][S] complex -> [N][S*factor] complex."""
+        d = _c128(data)
+        N, S = d.shape
+        out = np.empty((N, S * factor), np.complex128)
+        self._chk(self.L.bsar_upsample(self.h, _ptr(d.view(np.float64)), S, N, factor, _ptr(out.view(np.float64))),
+                  "upsample")
+        return out
+
+    def range_compress(self, sig, ref, conj: bool = True, precision: int = FP64) -> np.ndarray:
+        """sig [N][S] complex columns, ref [S] -> [N][S] correlation (conj) / convolution."""
+        s = _c128(sig)
+        N, S = s.shape
+        r = _c128(ref).reshape(-1)
+        if len(r) != S:
+            raise ValueError("reference length must equal the column length")
+        out = np.empty_like(s)
+        self._chk(self.L.bsar_range_compress(self.h, _ptr(s.view(np.float64)), S, N, _ptr(r.view(np.float64)),
+                                             int(conj), precision, _ptr(out.view(np.float64))), "range_compress")
+        return out
+
+    def dataset(self, params: Params, dem: Dem, tx, rx) -> "Dataset":
+        return Dataset(self, params, dem, tx, rx)
+
+
+class Dataset:
+    """bsar_dataset: DEM, antennas and (upsampled) range data resident on the GPU."""
+
+    def __init__(self, ctx: Context, params: Params, dem: Dem, tx, rx):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.tx = _f64(tx).reshape(-1, 3)
+        self.rx = _f64(rx).reshape(-1, 3)
+        if self.tx.shape != self.rx.shape:
+            raise ValueError("Tx and Rx need the same number of positions")
+        self.n = len(self.tx)
+        self.params = params
+        p = params.c_struct()
+        d = dem.c()
+        h = C.c_void_p()
+        ctx._chk(self.L.bsar_dataset_create(ctx.h, C.byref(p), C.byref(d), _ptr(self.tx), _ptr(self.rx), self.n,
+                                            C.byref(h)), "dataset_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bsar_dataset_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_rootdata(self, root):
+        """root [N_data][S] complex (range fastest), upsampled on the GPU."""
+        r = _c128(root)
+        nd, S = r.shape
+        self.ctx._chk(self.L.bsar_dataset_load_rootdata(self.ctx.h, self.h, _ptr(r.view(np.float64)), S, nd),
+                      "load_rootdata")
+
+    def load_upsampled(self, rc):
+        r = _c128(rc)
+        nd, S4 = r.shape
+        self.ctx._chk(self.L.bsar_dataset_load_upsampled(self.ctx.h, self.h, _ptr(r.view(np.float64)), S4, nd),
+                      "load_upsampled")
+
+    def device_bytes(self) -> int:
+        return int(self.L.bsar_dataset_device_bytes(self.h))
+
+    def slant_range_fermat(self, pts):
+        p = _f64(pts).reshape(-1, 3)
+        ida = np.empty((len(p), self.n, 3))
+        volta = np.empty_like(ida)
+        self.ctx._chk(self.L.bsar_slant_range_fermat(self.ctx.h, self.h, _ptr(p), len(p), _ptr(ida), _ptr(volta)),
+                      "slant_range_fermat")
+        return ida, volta
+
+    def backproject(self, gx, gy, gz, precision: int = FP64):
+        """Image [nz][ny][nx] complex (x fastest) and the run report (dict)."""
+        gx, gy, gz = _f64(gx), _f64(gy), _f64(gz)
+        out = np.empty((len(gz), len(gy), len(gx)), np.complex128)
+        rep = ReportC()
+        self.ctx._chk(self.L.bsar_backproject(self.ctx.h, self.h, _ptr(gx), len(gx), _ptr(gy), len(gy), _ptr(gz),
+                                              len(gz), precision, _ptr(out.view(np.float64)), C.byref(rep)),
+                      "backproject")
+        return out, {f: getattr(rep, f) for f, _ in ReportC._fields_}
+
+
+def chirp(f0: float, f1: float, T: float, fs: float) -> np.ndarray:
+    L = lib()
+    n = L.bsar_chirp(f0, f1, T, fs, None)
+    if n < 0:
+        raise BsarError(n, "chirp")
+    out = np.empty(n, np.complex128)
+    L.bsar_chirp(f0, f1, T, fs, _ptr(out.view(np.float64)))
+    return out
+
+
+def read_matrix(path) -> tuple[np.ndarray, list[int]]:
+    """readBinary<double>: flat complex array and header dims."""
+    L = lib()
+    dims = (C.c_uint64 * 64)()
+    nd, cplx, cnt = C.c_int(), C.c_int(), C.c_uint64()
+    st = L.bsar_read_matrix(str(path).encode(), dims, 64, C.byref(nd), C.byref(cplx), None, C.byref(cnt))
+    if st != 0:
+        raise BsarError(st, f"read_matrix({path})")
+    out = np.empty(cnt.value, np.complex128)
+    st = L.bsar_read_matrix(str(path).encode(), dims, 64, C.byref(nd), C.byref(cplx), _ptr(out.view(np.float64)),
+                            C.byref(cnt))
+    if st != 0:
+        raise BsarError(st, f"read_matrix({path})")
+    return out, [dims[i] for i in range(nd.value)]
+
+
+def write_image(path, img, gx=None, gy=None, gz=None):
+    """export3DComplexMatrix: img [nz][ny][nx] complex."""
+    L = lib()
+    im = _c128(img)
+    nz, ny, nx = im.shape
+    axes = [None if a is None else _f64(a) for a in (gx, gy, gz)]
+    ptrs = [None if a is None else _ptr(a) for a in axes]
+    st = L.bsar_write_image(str(path).encode(), _ptr(im.view(np.float64)), nx, ny, nz, *ptrs)
+    if st != 0:
+        raise BsarError(st, f"write_image({path})")
+
+
+def write_matrix2d(path, data):
+    L = lib()
+    d = _c128(data)
+    rows, cols = d.shape
+    st = L.bsar_write_matrix2d(str(path).encode(), _ptr(d.view(np.float64)), rows, cols)
+    if st != 0:
+        raise BsarError(st, f"write_matrix2d({path})")
+
+
+def flops_per_voxel_pulse(n: int) -> float:
+    return float(lib().bsar_flops_per_voxel_pulse(n))

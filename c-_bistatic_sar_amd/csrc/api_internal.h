@@ -1,0 +1,72 @@
+// api_internal.h -- host-side state behind the C ABI (bsar.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "bsar.h"
+#include "bsar_kernels.h"
+
+struct bsar_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct bsar_dataset {
+    bsar_ctx* owner = nullptr;
+    int device = 0;
+    bsar_params p{};
+    int n = 0;                       // trajectory pulses
+    // DEM on device
+    double* dX = nullptr;
+    double* dY = nullptr;
+    double* dZ = nullptr;
+    bsar::DemDev dem{};
+    double zmin = 0.0, zmax = 0.0;   // DEM height range (0 when empty)
+    // antennas (SoA) on device + host copy
+    double* dtraj = nullptr;         // 6 * n doubles
+    std::vector<double> htx, hrx;    // [n][3]
+    bsar::Traj traj{};
+    // upsampled range-compressed data
+    double2* rc = nullptr;
+    float2* rc32 = nullptr;
+    int ndata = 0, S4 = 0;
+    int64_t bytes = 0;
+    // R1-mean tables (mixed form), built lazily per dataset
+    void* r1tab = nullptr;
+};
+
+extern "C" void bsar_r1tab_free(bsar_dataset* ds);
+
+#define BSAR_CHECK_HIP(ctx, call)                                     \
+    do {                                                              \
+        hipError_t e_ = (call);                                       \
+        if (e_ != hipSuccess) return ::bsar::hip_fail((ctx), e_, #call); \
+    } while (0)
+
+namespace bsar {
+
+int set_error(bsar_ctx* ctx, int status, const std::string& msg);
+int hip_fail(bsar_ctx* ctx, hipError_t e, const char* what);
+
+// Builds the device DEM description from the caller's arrays (validation,
+// padding of missing Z entries with 0.0, sortedness flags, z range).
+int upload_dem(bsar_ctx* ctx, const bsar_dem* dem, double** dX, double** dY, double** dZ, DemDev* out,
+               double* zmin, double* zmax);
+
+// Mixed-precision back-projection (kernels_mixed.hip / mixed.cpp).
+int backproject_mixed(bsar_ctx* ctx, const bsar_dataset* ds, const double* gx, int nx, const double* gy, int ny,
+                      const double* gz, int nz, double2* d_out, bsar_report* rep);
+
+// Work-splitting of one back-projection over launches.
+struct BpPlan {
+    int groups = 1, glen = 0, wlen = 0;
+    int64_t nvox = 0, nvox_pad = 0;
+};
+BpPlan plan_bp(int64_t nvox, int n_pulses, double cost_per_vp);
+
+}  // namespace bsar

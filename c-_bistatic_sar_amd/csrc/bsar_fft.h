@@ -1,0 +1,23 @@
+// bsar_fft.h -- batched FFT filtering (range compression) launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace bsar {
+
+// Columns whose complex samples fit in this many LDS bytes run in one
+// workgroup each; larger ones run as global radix-2 passes.
+constexpr size_t kFftLdsMax = 128 * 1024;
+
+// data[c] <- IFFT(FFT(data[c]) .* H) for ncols contiguous columns of length
+// S (power of two).  tmp: ncols*S scratch, used only by the global path.
+hipError_t fft_filter(double2* data, double2* tmp, const double2* H, int S, int64_t ncols, hipStream_t st);
+hipError_t fft_filter(float2* data, float2* tmp, const float2* H, int S, int64_t ncols, hipStream_t st);
+// Forward FFT of ncols columns (global passes); *result is data or tmp.
+hipError_t fft_forward(double2* data, double2* tmp, int S, int64_t ncols, hipStream_t st, double2** result);
+hipError_t to_f32(const double2* a, float2* b, int64_t n, hipStream_t st);
+hipError_t to_f64(const float2* a, double2* b, int64_t n, hipStream_t st);
+hipError_t conj_inplace(double2* a, int64_t n, hipStream_t st);
+
+}  // namespace bsar

@@ -1,0 +1,54 @@
+// bsar_kernels.h -- argument blocks and host launchers of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bsar_device.h"
+
+namespace bsar {
+
+// DEMs up to this many doubles (X + Y + padded Z) are staged in LDS per
+// workgroup; larger ones are read through the L1/L2 caches.
+constexpr int kLdsDemMax = 8192;   // 64 KiB
+
+// Antenna positions, structure of arrays, resident in HBM.
+struct Traj {
+    const double* tx; const double* ty; const double* tz;
+    const double* rx; const double* ry; const double* rz;
+    int n;
+};
+
+// One back-projection launch: voxels [vox0, vox0 + nvox) of an nx*ny*nz
+// grid (x fastest), pulse groups g = 0..groups-1 each covering pulses
+// [g*glen, min((g+1)*glen, n)), of which this launch adds the window
+// [woff, woff + wlen) to the group's running sums acc[g][v].
+struct BpArgs {
+    DemDev dem;
+    Traj traj;
+    const double2* rc;      // upsampled range-compressed data [ndata][S4]
+    const float2* rc32;     // same in FP32 (mixed form), may be null
+    int ndata, S4;
+    double s4m1;            // S4 - 1 as double (window test)
+    double n1, n2;
+    double inv_c;           // 1.0 / c              (proc_test_script.cpp:567)
+    double fs4;             // fs * upsample        (proc_test_script.cpp:223)
+    double kphi;            // 2.0 * M_PI / lambda  (proc_test_script.cpp:575)
+    const double* gx; const double* gy; const double* gz;
+    int nx, ny, nz;
+    int64_t vox0, nvox, nvox_pad;
+    int groups, glen, woff, wlen;
+    double2* acc;           // [groups][acc_stride]
+    int64_t acc_stride;
+};
+
+hipError_t launch_dem_interp(const DemDev& dem, const double* xy, int64_t m, double* z, hipStream_t s);
+hipError_t launch_refraction(const DemDev& dem, const double* legs, int64_t m, double n1, double n2,
+                             double* out, hipStream_t s);
+hipError_t launch_slant_range(const DemDev& dem, const Traj& tr, const double* pts, int64_t V, double n1,
+                              double n2, double* ida, double* volta, hipStream_t s);
+hipError_t launch_spline(const double2* in, int S, int N, int f, double2* out, hipStream_t s);
+hipError_t launch_bp_fp64(const BpArgs& a, hipStream_t s);
+hipError_t launch_group_sum(const double2* acc, int groups, int64_t stride, int64_t nvox, double2* out,
+                            hipStream_t s);
+
+}  // namespace bsar

@@ -1,0 +1,259 @@
+// kernels_fp64.hip -- FP64 kernels of the imaging hot path (gfx950).
+//
+//   k_dem_interp      interpolateDEMElevation, batched          (raw_data_processing.cpp:615-680)
+//   k_refraction      calculateRefractionPoint, batched legs    (raw_data_processing.cpp:455-572)
+//   k_slant_range     refraction points of calculaSlantRangeFermat (raw_data_processing.cpp:153-193)
+//   k_spline          splineInterpolation x factor per pulse    (interpolation.cpp:4-51)
+//   k_bp_fp64         fused plain-FP64 back-projection          (proc_test_script.cpp:484-668)
+//   k_group_sum       ordered sum of pulse-group partial images
+//
+// Built with -ffp-contract=off: the plain path evaluates the reference's
+// operation order exactly, so refraction points, distances, delays and the
+// interpolation weights are bit-identical to the reference's FP64 CPU build;
+// only sin/cos (device libm vs glibc) may differ in the last bit.
+#include "bsar_device.h"
+#include "bsar_kernels.h"
+
+#include <math.h>
+
+namespace bsar {
+
+__global__ __launch_bounds__(256) void k_dem_interp(DemDev dem, const double* __restrict__ xy, int64_t m,
+                                                    double* __restrict__ z)
+{
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    z[k] = dem_interp(dem, xy[2 * k], xy[2 * k + 1]);
+}
+
+__global__ __launch_bounds__(256) void k_refraction(DemDev dem, const double* __restrict__ legs, int64_t m,
+                                                    double n1, double n2, double* __restrict__ out)
+{
+    extern __shared__ double lds[];
+    DemDev d = dem;
+    if (dem.nx + dem.ny + dem.ny * dem.zs <= kLdsDemMax) {
+        d = dem_to_lds(dem, lds);
+        __syncthreads();
+    }
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const double* s = legs + 6 * k;
+    double qx, qy, qz;
+    fermat_solve(d, s[0], s[1], s[2], s[3], s[4], s[5], n1, n2, qx, qy, qz);
+    out[3 * k] = qx;
+    out[3 * k + 1] = qy;
+    out[3 * k + 2] = qz;
+}
+
+// Thread per (point v, pulse i): both legs of one calculaSlantRangeFermat
+// iteration (raw_data_processing.cpp:166-181).
+__global__ __launch_bounds__(256) void k_slant_range(DemDev dem, Traj tr, const double* __restrict__ pts,
+                                                     int64_t V, double n1, double n2,
+                                                     double* __restrict__ ida, double* __restrict__ volta)
+{
+    extern __shared__ double lds[];
+    DemDev d = dem;
+    if (dem.nx + dem.ny + dem.ny * dem.zs <= kLdsDemMax) {
+        d = dem_to_lds(dem, lds);
+        __syncthreads();
+    }
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= V * tr.n) return;
+    int64_t v = k / tr.n;
+    int i = (int)(k - v * tr.n);
+    const double Px = pts[3 * v], Py = pts[3 * v + 1], Pz = pts[3 * v + 2];
+    double qx, qy, qz;
+    fermat_solve(d, tr.tx[i], tr.ty[i], tr.tz[i], Px, Py, Pz, n1, n2, qx, qy, qz);
+    ida[3 * k] = qx; ida[3 * k + 1] = qy; ida[3 * k + 2] = qz;
+    fermat_solve(d, Px, Py, Pz, tr.rx[i], tr.ry[i], tr.rz[i], n2, n1, qx, qy, qz);
+    volta[3 * k] = qx; volta[3 * k + 1] = qy; volta[3 * k + 2] = qz;
+}
+
+// splineInterpolation, interpolation.cpp:4-51: one thread per output sample.
+// in: [N][S] complex, out: [N][S*f] complex.
+__global__ __launch_bounds__(256) void k_spline(const double2* __restrict__ in, int S, int N, int f,
+                                                double2* __restrict__ out)
+{
+    const int64_t S4 = (int64_t)S * f;
+    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= S4 * N) return;
+    const int64_t p = k / S4;
+    const int64_t i = k - p * S4;
+    const double2* d = in + p * S;
+    double pos = (double)i / f;                                       // :18
+    int64_t idx = (int64_t)pos;                                       // :19
+    double frac = pos - (double)idx;                                  // :20
+    if (idx >= S - 1) { idx = S - 2; frac = 1.0; }                    // :22-25
+    const double t = frac, t2 = t * t, t3 = t2 * t;
+    const double h1 = 2 * t3 - 3 * t2 + 1;                            // :33-36
+    const double h2 = -2 * t3 + 3 * t2;
+    const double h3 = t3 - 2 * t2 + t;
+    const double h4 = t3 - t2;
+    const double2 a = d[idx], b = d[idx + 1];
+    double2 dp = make_double2(0.0, 0.0), dn = make_double2(0.0, 0.0);
+    if (idx > 0) { double2 c = d[idx - 1]; dp = make_double2(a.x - c.x, a.y - c.y); }          // :39,41
+    if (idx < S - 2) { double2 c = d[idx + 2]; dn = make_double2(c.x - b.x, c.y - b.y); }      // :40,42
+    double2 r;
+    r.x = h1 * a.x + h2 * b.x + h3 * dp.x + h4 * dn.x;                // :45
+    r.y = h1 * a.y + h2 * b.y + h3 * dp.y + h4 * dn.y;                // :46
+    out[k] = r;
+}
+
+// Plain FP64 back-projection.  Thread = (pulse group g, voxel v); it carries
+// the running sum of its voxel across launches (acc), adding the pulses of
+// this launch's window in increasing order -- with one group that is exactly
+// the reference's sequential accumulation (proc_test_script.cpp:589-641).
+template <bool kLds>
+__global__ __launch_bounds__(256) void k_bp_fp64(BpArgs a)
+{
+    extern __shared__ double lds[];
+    DemDev d = a.dem;
+    if (kLds) {
+        d = dem_to_lds(a.dem, lds);
+        __syncthreads();
+    }
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    // nvox_pad is a multiple of the wavefront size: g is wave-uniform.
+    const int g = __builtin_amdgcn_readfirstlane((int)(t / a.nvox_pad));
+    const int64_t vl = t - (int64_t)g * a.nvox_pad;
+    if (g >= a.groups || vl >= a.nvox) return;
+    const int gbeg = g * a.glen;
+    const int gend = min(gbeg + a.glen, a.traj.n);
+    const int p0 = gbeg + a.woff;
+    const int p1 = min(p0 + a.wlen, gend);
+    if (p0 >= p1) return;
+    const int64_t v = a.vox0 + vl;
+    const int xi = (int)(v % a.nx);
+    const int yi = (int)((v / a.nx) % a.ny);
+    const int zi = (int)(v / ((int64_t)a.nx * a.ny));
+    const double Px = a.gx[xi], Py = a.gy[yi], Pz = a.gz[zi];
+    const double n1 = a.n1, n2 = a.n2;
+    const int n = a.traj.n;
+    const double dn = (double)n;
+    double2* accp = a.acc + (int64_t)g * a.acc_stride + v;
+    double accr = accp->x, acci = accp->y;
+
+    for (int i = p0; i < p1; ++i) {
+        double qax, qay, qaz, qbx, qby, qbz;
+        fermat_solve(d, a.traj.tx[i], a.traj.ty[i], a.traj.tz[i], Px, Py, Pz, n1, n2, qax, qay, qaz);   // rdp:166-172
+        fermat_solve(d, Px, Py, Pz, a.traj.rx[i], a.traj.ry[i], a.traj.rz[i], n2, n1, qbx, qby, qbz);   // rdp:175-181
+        double e0 = Px - qax, e1 = Py - qay, e2 = Pz - qaz;
+        const double R2t = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:496-505
+        e0 = Px - qbx; e1 = Py - qby; e2 = Pz - qbz;
+        const double R2r = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:507-516
+        double st = 0.0, sr = 0.0;
+        for (int j = 0; j < n; ++j) {                                                                     // proc:518-532
+            const double f0 = a.traj.tx[j] - qax, f1 = a.traj.ty[j] - qay, f2 = a.traj.tz[j] - qaz;
+            st += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+        }
+        for (int j = 0; j < n; ++j) {                                                                     // proc:534-548
+            const double f0 = a.traj.rx[j] - qbx, f1 = a.traj.ry[j] - qby, f2 = a.traj.rz[j] - qbz;
+            sr += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+        }
+        const double R1t = st / dn, R1r = sr / dn;
+        const double path = n1 * R1t + n2 * R2t + n2 * R2r + n1 * R1r;
+        const double tv = a.inv_c * path;                                                                // proc:567
+        const double rngBin = 1.0 + (tv * a.fs4);                                                        // proc:571
+        const double phi = a.kphi * path;                                                                // proc:575
+        const double r = rngBin - 1.0;                                                                   // proc:593
+        if (!(r >= 0.0 && r <= a.s4m1)) continue;                                                        // proc:595-599
+        const int iL = (int)floor(r);
+        const int iN = (int)ceil(r);
+        const double q = r - (double)iL;                                                                 // proc:602
+        const int az = min(i, a.ndata - 1);                                                              // proc:607
+        const double2* row = a.rc + (int64_t)az * a.S4;
+        const double2 L = row[iL];
+        double vr, vi;
+        if (iL == iN) {                                                                                  // proc:626-628
+            vr = L.x; vi = L.y;
+        } else {                                                                                         // proc:632-634
+            const double2 N = row[iN];
+            vr = (1.0 - q) * L.x + q * N.x;
+            vi = (1.0 - q) * L.y + q * N.y;
+        }
+        double sn, cs;
+        sincos(phi, &sn, &cs);                                                                           // proc:638
+        accr += vr * cs - vi * sn;                                                                       // proc:641
+        acci += vr * sn + vi * cs;
+    }
+    *accp = make_double2(accr, acci);
+}
+
+// out[v] = sum over groups in order; out is [nvox] complex (x-fastest voxels).
+__global__ __launch_bounds__(256) void k_group_sum(const double2* __restrict__ acc, int groups, int64_t stride,
+                                                   int64_t nvox, double2* __restrict__ out)
+{
+    int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    double r = 0.0, m = 0.0;
+    for (int g = 0; g < groups; ++g) {
+        const double2 x = acc[(int64_t)g * stride + v];
+        r += x.x;
+        m += x.y;
+    }
+    out[v] = make_double2(r, m);
+}
+
+// ---- host launchers -------------------------------------------------------
+
+static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+static inline size_t dem_lds_bytes(const DemDev& d)
+{
+    const int cnt = d.nx + d.ny + d.ny * d.zs;
+    return cnt <= kLdsDemMax ? (size_t)cnt * sizeof(double) : 0;
+}
+
+hipError_t launch_dem_interp(const DemDev& dem, const double* xy, int64_t m, double* z, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dem_interp, dim3(blocks_for(m, 256)), dim3(256), 0, s, dem, xy, m, z);
+    return hipGetLastError();
+}
+
+hipError_t launch_refraction(const DemDev& dem, const double* legs, int64_t m, double n1, double n2,
+                             double* out, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_refraction, dim3(blocks_for(m, 256)), dim3(256), dem_lds_bytes(dem), s,
+                       dem, legs, m, n1, n2, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_slant_range(const DemDev& dem, const Traj& tr, const double* pts, int64_t V, double n1,
+                              double n2, double* ida, double* volta, hipStream_t s)
+{
+    const int64_t m = V * tr.n;
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_slant_range, dim3(blocks_for(m, 256)), dim3(256), dem_lds_bytes(dem), s,
+                       dem, tr, pts, V, n1, n2, ida, volta);
+    return hipGetLastError();
+}
+
+hipError_t launch_spline(const double2* in, int S, int N, int f, double2* out, hipStream_t s)
+{
+    const int64_t m = (int64_t)S * f * N;
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_spline, dim3(blocks_for(m, 256)), dim3(256), 0, s, in, S, N, f, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_fp64(const BpArgs& a, hipStream_t s)
+{
+    const int64_t threads = (int64_t)a.groups * a.nvox_pad;
+    if (threads <= 0) return hipSuccess;
+    const size_t lds = dem_lds_bytes(a.dem);
+    if (lds) hipLaunchKernelGGL(k_bp_fp64<true>, dim3(blocks_for(threads, 256)), dim3(256), lds, s, a);
+    else hipLaunchKernelGGL(k_bp_fp64<false>, dim3(blocks_for(threads, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_sum(const double2* acc, int groups, int64_t stride, int64_t nvox, double2* out,
+                            hipStream_t s)
+{
+    if (nvox <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_group_sum, dim3(blocks_for(nvox, 256)), dim3(256), 0, s, acc, groups, stride, nvox, out);
+    return hipGetLastError();
+}
+
+}  // namespace bsar

@@ -1,0 +1,87 @@
+// rc.cpp -- bsar_range_compress: batched FFT correlation / convolution of
+// pulse columns against one reference (fft_utils.cpp:63-137 as driven by
+// raw_data_processing.cpp:406-446).
+#include <cstring>
+#include <vector>
+
+#include "api_internal.h"
+#include "bsar_fft.h"
+
+namespace {
+
+struct Buf {
+    void* p = nullptr;
+    ~Buf() { if (p) (void)hipFree(p); }
+};
+
+size_t next_pow2(size_t n)   // fft_utils.cpp:187-195
+{
+    size_t p = 1;
+    while (p < n) p *= 2;
+    return p;
+}
+
+}  // namespace
+
+using namespace bsar;
+
+extern "C" int bsar_range_compress(bsar_ctx* c, const double* sig, int S, int N, const double* ref, int conj,
+                                   int precision, double* out)
+{
+    if (!c) return BSAR_ERR_INVALID;
+    if (S < 1 || N < 0 || (N && (!sig || !out)) || !ref)
+        return set_error(c, BSAR_ERR_INVALID, "range_compress needs S >= 1, N >= 0 and buffers");
+    if (precision != BSAR_FP64 && precision != BSAR_MIXED) return set_error(c, BSAR_ERR_INVALID, "bad precision");
+    BSAR_CHECK_HIP(c, hipSetDevice(c->device));
+    if (N == 0) return BSAR_OK;
+    // Both inputs are zero-padded to the next power of two and the first S
+    // samples of the circular result are kept (fft_utils.cpp:68-78, 95-98).
+    const size_t P = next_pow2((size_t)S);
+    if (P > (size_t)1 << 26) return set_error(c, BSAR_ERR_INVALID, "column too long");
+    std::vector<double> padded;
+    const double* src = sig;
+    if (P != (size_t)S) {
+        padded.assign(2 * P * N, 0.0);
+        for (int k = 0; k < N; ++k) std::memcpy(&padded[2 * P * k], sig + 2 * (size_t)S * k, 16 * (size_t)S);
+        src = padded.data();
+    }
+    std::vector<double> refp(2 * P, 0.0);
+    std::memcpy(refp.data(), ref, 16 * (size_t)S);
+
+    const size_t n = P * (size_t)N;
+    Buf dref, dtmp1, data, tmp;
+    BSAR_CHECK_HIP(c, hipMalloc(&dref.p, P * sizeof(double2)));
+    BSAR_CHECK_HIP(c, hipMalloc(&dtmp1.p, P * sizeof(double2)));
+    BSAR_CHECK_HIP(c, hipMemcpyAsync(dref.p, refp.data(), P * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    double2* H = nullptr;
+    BSAR_CHECK_HIP(c, fft_forward((double2*)dref.p, (double2*)dtmp1.p, (int)P, 1, c->stream, &H));
+    if (conj) BSAR_CHECK_HIP(c, conj_inplace(H, (int64_t)P, c->stream));
+    const bool need_tmp_fp64 = P * sizeof(double2) > kFftLdsMax;
+    const bool need_tmp_fp32 = P * sizeof(float2) > kFftLdsMax;
+    BSAR_CHECK_HIP(c, hipMalloc(&data.p, n * sizeof(double2)));
+    BSAR_CHECK_HIP(c, hipMemcpyAsync(data.p, src, n * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    if (precision == BSAR_FP64) {
+        if (need_tmp_fp64) BSAR_CHECK_HIP(c, hipMalloc(&tmp.p, n * sizeof(double2)));
+        BSAR_CHECK_HIP(c, fft_filter((double2*)data.p, (double2*)tmp.p, H, (int)P, N, c->stream));
+    } else {
+        Buf d32, h32;
+        BSAR_CHECK_HIP(c, hipMalloc(&d32.p, n * sizeof(float2)));
+        BSAR_CHECK_HIP(c, hipMalloc(&h32.p, P * sizeof(float2)));
+        if (need_tmp_fp32) BSAR_CHECK_HIP(c, hipMalloc(&tmp.p, n * sizeof(float2)));
+        BSAR_CHECK_HIP(c, to_f32((const double2*)data.p, (float2*)d32.p, (int64_t)n, c->stream));
+        BSAR_CHECK_HIP(c, to_f32(H, (float2*)h32.p, (int64_t)P, c->stream));
+        BSAR_CHECK_HIP(c, fft_filter((float2*)d32.p, (float2*)tmp.p, (const float2*)h32.p, (int)P, N, c->stream));
+        BSAR_CHECK_HIP(c, to_f64((const float2*)d32.p, (double2*)data.p, (int64_t)n, c->stream));
+        BSAR_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    if (P == (size_t)S) {
+        BSAR_CHECK_HIP(c, hipMemcpyAsync(out, data.p, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        BSAR_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<double> full(2 * n);
+        BSAR_CHECK_HIP(c, hipMemcpyAsync(full.data(), data.p, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        BSAR_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+        for (int k = 0; k < N; ++k) std::memcpy(out + 2 * (size_t)S * k, &full[2 * P * k], 16 * (size_t)S);
+    }
+    return BSAR_OK;
+}

@@ -1,0 +1,266 @@
+"""Seeded synthetic scenes for the imaging workloads, and the forward model
+that turns them into range-compressed data on the GPU.
+
+Geometry and signal follow the reference's simulator (gs_test_script.cpp:
+15-137, processRawData raw_data_processing.cpp:292-452): per pulse the
+bistatic refraction path of each scatterer gives a delay bin
+1 + round(t*fs) and phase -(2*pi/lambda)*path; unit responses are convolved
+with the fftshifted chirp and correlated with it (pulse compression).  The
+refraction points come from the library's FP64 Fermat kernel and the
+convolution/correlation from its batched FFT kernel.
+
+Everything that the reference writes to CSV (antenna positions, DEM, grid
+axes) is quantised through "%.6f" text, as data_export.cpp:148,169 does, so
+in-memory runs and file-based runs see the same doubles.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def q6(a) -> np.ndarray:
+    """Round through the reference's "%.6f" CSV text (std::fixed, setprecision(6))."""
+    a = np.asarray(a, dtype=np.float64)
+    return np.array([float("%.6f" % v) for v in a.ravel()]).reshape(a.shape)
+
+
+def spiral(n: int, r0: float = 5.0, r1: float = 15.0, turns: float = 2.0, h: float = 12.0,
+           phase: float = 0.0, x0: float = 0.0, y0: float = 0.0) -> np.ndarray:
+    """Archimedean spiral, radius r0 -> r1 over `turns` turns at height h: [n][3]."""
+    p = np.arange(n, dtype=np.float64)
+    s = p / max(n - 1, 1)
+    r = r0 + (r1 - r0) * s
+    th = 2.0 * np.pi * turns * s + phase
+    return q6(np.stack([x0 + r * np.cos(th), y0 + r * np.sin(th), np.full(n, h)], 1))
+
+
+def dem_bumps(rng: np.random.Generator, n: int = 34, spacing: float = 30.0, bumps: int = 5,
+              amp=(0.5, 2.0), sigma=(60.0, 150.0), slope: float = 0.01):
+    """Plane plus Gaussian bumps on an n x n grid centred on the origin."""
+    ax = -(n - 1) * spacing / 2.0 + spacing * np.arange(n)
+    X, Y = np.meshgrid(ax, ax, indexing="xy")
+    a, b = rng.uniform(-slope, slope, 2)
+    Z = a * X + b * Y
+    for _ in range(bumps):
+        A = rng.uniform(*amp)
+        s = rng.uniform(*sigma)
+        cx, cy = rng.uniform(ax[0], ax[-1], 2)
+        Z = Z + A * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * s * s))
+    return q6(ax), q6(ax), q6(Z)
+
+
+def dem_fbm(rng: np.random.Generator, nx: int = 167, ny: int = 100, spacing: float = 30.0, hurst: float = 0.7,
+            rms: float = 40.0):
+    """Fractional-Brownian relief by spectral synthesis (|k|^-(2H+2) power)."""
+    kx = np.fft.fftfreq(nx)[None, :]
+    ky = np.fft.fftfreq(ny)[:, None]
+    k = np.sqrt(kx * kx + ky * ky)
+    k[0, 0] = 1.0
+    amp = k ** (-(hurst + 1.0))
+    amp[0, 0] = 0.0
+    spec = amp * (rng.normal(size=(ny, nx)) + 1j * rng.normal(size=(ny, nx)))
+    Z = np.real(np.fft.ifft2(spec))
+    Z *= rms / np.sqrt(np.mean(Z * Z))
+    x = -(nx - 1) * spacing / 2.0 + spacing * np.arange(nx)
+    y = -(ny - 1) * spacing / 2.0 + spacing * np.arange(ny)
+    return q6(x), q6(y), q6(Z)
+
+
+def axis(start: float, step: float, count: int) -> np.ndarray:
+    return q6(start + step * np.arange(count))
+
+
+@dataclass
+class Scene:
+    name: str
+    fc: float
+    c: float
+    fs: float            # ADC rate of the raw data (before the x4 upsampling)
+    bw: float
+    T: float
+    n1: float
+    n2: float
+    S: int
+    tx: np.ndarray
+    rx: np.ndarray
+    dem: tuple
+    gx: np.ndarray
+    gy: np.ndarray
+    gz: np.ndarray
+    scat: np.ndarray
+    rcs: np.ndarray
+    snr_db: float | None = None
+    seed: int = 0
+    upsample: int = 4
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return len(self.tx)
+
+    @property
+    def voxels(self) -> int:
+        return len(self.gx) * len(self.gy) * len(self.gz)
+
+    def sim_params(self) -> dict:
+        return dict(fc=self.fc, c=self.c, fs=self.fs, f0=-self.bw / 2, f1=self.bw / 2, T=self.T, fs_chirp=self.fs,
+                    n1=self.n1, n2=self.n2, S=self.S)
+
+    def bp_params(self) -> dict:
+        return dict(c=self.c, fs4=self.fs * self.upsample, lam=self.c / self.fc, n1=self.n1, n2=self.n2)
+
+
+BOX = dict(x=(19.2, 20.8), y=(-0.8, 0.8), z=(-5.8, -4.2))
+
+
+def _grid(nx, ny, nz, box=BOX):
+    def ax(lo, hi, m):
+        return axis(lo, (hi - lo) / (m - 1) if m > 1 else 0.0, m)
+    return ax(*box["x"], nx), ax(*box["y"], ny), ax(*box["z"], nz)
+
+
+def _scatterers(rng, count, box=BOX, margin=0.1):
+    lo = np.array([box["x"][0], box["y"][0], box["z"][0]]) + margin
+    hi = np.array([box["x"][1], box["y"][1], box["z"][1]]) - margin
+    return rng.uniform(lo, hi, size=(count, 3))
+
+
+def make_scene(name: str, seed: int | None = None, n_pulses: int | None = None, grid=None, samples=None,
+               clutter: int | None = None) -> Scene:
+    """Workloads:
+      small   41x41x21 grid, 256 pulses, 4096 samples, 3 scatterers, 20 dB SNR
+      large   201x201x101 grid, 2048 pulses, 8192 samples, 16 scatterers + clutter layer
+      rough   128^3 grid 4x4x4 m buried 3-7 m under fBm terrain (167x100 DEM), 1024 pulses
+      wide    512x512x64 grid, 4096 pulses, 16384 samples, 64 scatterers, B = 500 MHz
+    Every one: fc = 1 GHz, T = 1 us, fs = 800 MHz, c = 3e8, n1 = 1, n2 = sqrt(eps_r).
+    """
+    seeds = dict(small=1001, large=1002, rough=1003, wide=1004, serve=1005)
+    seed = seeds.get(name, 1000) if seed is None else seed
+    rng = np.random.default_rng(seed)
+    fc, c, fs, T = 1e9, 3e8, 800e6, 1e-6
+    if name == "small":
+        N, S, bw, er = 256, 4096, 300e6, 6.0
+        gx, gy, gz = _grid(41, 41, 21)
+        dem = dem_bumps(rng)
+        scat = _scatterers(rng, 3)
+        rcs = np.ones(3, np.complex128)
+        snr = 20.0
+    elif name == "large":
+        N, S, bw, er = 2048, 8192, 300e6, 6.0
+        gx, gy, gz = _grid(201, 201, 101)
+        dem = dem_bumps(rng)
+        pts = _scatterers(rng, 16)
+        ncl = 256 if clutter is None else clutter
+        # clutter layer 0.5 m thick at the top of the box, uniform amplitude, random phase
+        cl = np.stack([rng.uniform(*BOX["x"], ncl), rng.uniform(*BOX["y"], ncl),
+                       rng.uniform(BOX["z"][1] - 0.5, BOX["z"][1], ncl)], 1)
+        scat = np.concatenate([pts, cl])
+        rcs = np.concatenate([np.ones(16), 0.1 * np.exp(2j * np.pi * rng.uniform(size=ncl))]).astype(np.complex128)
+        snr = None
+    elif name == "rough":
+        N, S, bw = 1024, 4096, 300e6
+        er = float(rng.uniform(4.0, 9.0))
+        dem = dem_fbm(rng)
+        # grid centre above the DEM's origin; buried 3-7 m below the surface there
+        zs = float(np.interp(0.0, dem[0], dem[2][np.searchsorted(dem[1], 0.0)]))
+        box = dict(x=(-2.0, 2.0), y=(-2.0, 2.0), z=(zs - 7.0, zs - 3.0))
+        gx, gy, gz = _grid(128, 128, 128, box)
+        scat = _scatterers(rng, 3, box)
+        rcs = np.ones(3, np.complex128)
+        snr = None
+        tx0 = spiral(N, h=zs + 12.0)
+        rx0 = spiral(N, h=zs + 12.0, phase=np.pi / 2)
+    elif name == "wide":
+        N, S, bw, er = 4096, 16384, 500e6, 6.0
+        gx, gy, gz = _grid(512, 512, 64)
+        dem = dem_bumps(rng)
+        scat = _scatterers(rng, 64)
+        rcs = np.ones(64, np.complex128)
+        snr = None
+    else:
+        raise ValueError(f"unknown workload {name!r}")
+    if n_pulses is not None:
+        N = n_pulses
+    if samples is not None:
+        S = samples
+    if grid is not None:
+        box = BOX if name != "rough" else dict(x=(gx[0], gx[-1]), y=(gy[0], gy[-1]), z=(gz[0], gz[-1]))
+        gx, gy, gz = _grid(*grid, box)
+    if name == "rough":
+        tx = tx0 if n_pulses is None else spiral(N, h=float(tx0[0, 2]))
+        rx = rx0 if n_pulses is None else spiral(N, h=float(rx0[0, 2]), phase=np.pi / 2)
+    else:
+        tx = spiral(N)
+        rx = spiral(N, phase=np.pi / 2)
+    return Scene(name=name, fc=fc, c=c, fs=fs, bw=bw, T=T, n1=1.0, n2=float(np.sqrt(er)), S=S, tx=tx, rx=rx,
+                 dem=dem, gx=gx, gy=gy, gz=gz, scat=scat, rcs=rcs, snr_db=snr, seed=seed)
+
+
+def reference_signal(chirp: np.ndarray, S: int) -> np.ndarray:
+    """fftshifted, zero-padded chirp of raw_data_processing.cpp:371-394."""
+    L = len(chirp)
+    if L > S:
+        raise ValueError("chirp longer than the range window")
+    half = L // 2
+    ref = np.zeros(S, np.complex128)
+    ref[: L - half] = chirp[half:]
+    ref[S - half:] = chirp[:half]
+    return ref
+
+
+def place_responses(scene: Scene, ida: np.ndarray, volta: np.ndarray) -> np.ndarray:
+    """auxData as [N][S] (pulse-major): delays and phases of
+    raw_data_processing.cpp:322-361 from refraction points ida/volta
+    [ns][N][3]; responses of several scatterers add."""
+    ns, N = ida.shape[:2]
+    P = scene.scat[:, None, :]
+    T = scene.tx[None, :, :]
+    R = scene.rx[None, :, :]
+    d = lambda a, b: np.sqrt(((a - b) ** 2).sum(-1))
+    r1Tx, r2Tx = d(T, ida), d(P, ida)
+    r1Rx, r2Rx = d(R, volta), d(P, volta)
+    total = scene.n1 * r1Tx + scene.n2 * r2Tx + scene.n1 * r1Rx + scene.n2 * r2Rx
+    t = total / scene.c
+    rbin = 1 + np.round(t * scene.fs).astype(np.int64)
+    lam = scene.c / scene.fc
+    phi = -(2.0 * np.pi / lam) * total
+    resp = scene.rcs[:, None] * np.exp(1j * phi)
+    aux = np.zeros((N, scene.S), np.complex128)
+    ok = (rbin >= 0) & (rbin < scene.S)
+    pulse = np.broadcast_to(np.arange(N)[None, :], rbin.shape)
+    np.add.at(aux, (pulse[ok], rbin[ok]), resp[ok])
+    return aux
+
+
+def simulate(ctx, scene: Scene, precision: int = 0) -> np.ndarray:
+    """Range-compressed data [N][S] (range fastest, the layout BP reads) on
+    the GPU: refraction points (Fermat kernel), response placement, chirp
+    convolution, optional complex Gaussian noise at scene.snr_db (power
+    relative to the mean raw-signal power), then pulse compression."""
+    import importlib.util
+    import sys
+    from pathlib import Path
+    pkg = sys.modules.get("bsar_amd")
+    if pkg is None:
+        spec = importlib.util.spec_from_file_location("bsar_amd", Path(__file__).parent / "__init__.py")
+        pkg = importlib.util.module_from_spec(spec)
+        sys.modules["bsar_amd"] = pkg
+        spec.loader.exec_module(pkg)
+    params = pkg.Params(scene.fc, scene.c, scene.fs, scene.n1, scene.n2, scene.upsample)
+    ds = ctx.dataset(params, pkg.Dem(*scene.dem), scene.tx, scene.rx)
+    try:
+        ida, volta = ds.slant_range_fermat(scene.scat)
+    finally:
+        ds.close()
+    aux = place_responses(scene, ida, volta)
+    chirp = pkg.chirp(-scene.bw / 2, scene.bw / 2, scene.T, scene.fs)
+    ref = reference_signal(chirp, scene.S)
+    raw = ctx.range_compress(aux, ref, conj=False, precision=precision)
+    if scene.snr_db is not None:
+        rng = np.random.default_rng(scene.seed + 7)
+        p = np.mean(np.abs(raw) ** 2)
+        sigma = np.sqrt(p / 10 ** (scene.snr_db / 10) / 2)
+        raw = raw + sigma * (rng.normal(size=raw.shape) + 1j * rng.normal(size=raw.shape))
+    return ctx.range_compress(raw, ref, conj=True, precision=precision)

@@ -1,0 +1,202 @@
+/*
+ * bsar.h -- C ABI of the MI355X-native bistatic-SAR imaging hot path.
+ *
+ * The reference (DavidAtenciaMondragon/c-_bistatic_sar) exposes this path as
+ * C++ free functions and an inline OpenMP loop; each entry point below names
+ * the reference interface it replaces (file:line).  Plain pointers and sizes
+ * only: no C++ or torch types cross this boundary.
+ *
+ * Conventions
+ *   - complex arrays are interleaved (re, im) float64, as in the reference's
+ *     std::complex<double> vectors and its binary files;
+ *   - every call returns BSAR_OK (0) or a negative bsar_status; the message
+ *     of the last failure on a context is bsar_last_error(ctx);
+ *   - a context owns one HIP stream on one GPU and is not thread-safe: use
+ *     one context per host thread (the reference runs these functions from
+ *     OpenMP threads, proc_test_script.cpp:451);
+ *   - host buffers are owned by the caller; device memory is owned by the
+ *     context or the dataset that allocated it.
+ */
+#ifndef BSAR_H
+#define BSAR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSAR_ABI_VERSION 1
+
+#if defined(__GNUC__)
+#define BSAR_API __attribute__((visibility("default")))
+#else
+#define BSAR_API
+#endif
+
+typedef enum {
+    BSAR_OK = 0,
+    BSAR_ERR_INVALID = -1,   /* bad argument, shape or size */
+    BSAR_ERR_DEVICE = -2,    /* HIP runtime error or no device */
+    BSAR_ERR_NOMEM = -3,     /* device or host allocation failed */
+    BSAR_ERR_IO = -4,        /* file could not be opened, read or written */
+    BSAR_ERR_FORMAT = -5,    /* malformed binary/CSV file */
+    BSAR_ERR_CAPACITY = -6,  /* serving: job does not fit the HBM cap */
+    BSAR_ERR_STATE = -7      /* call out of order (e.g. no range data loaded) */
+} bsar_status;
+
+typedef enum {
+    BSAR_FP64 = 0,   /* plain form: every step in FP64, reference operation order */
+    BSAR_MIXED = 1   /* FP64 refraction solve + delay/phase, FP32 interpolation,
+                        Kahan-compensated FP32 accumulation, tabulated R1 means */
+} bsar_precision;
+
+/* Radar/system/environment constants that the reference fixes at compile
+ * time (radar_init.cpp:5-89) or reads from environment.txt
+ * (raw_data_processing.cpp:14-46).  Runtime values here. */
+typedef struct {
+    double fc;       /* carrier [Hz]            RadarTx.FreqPortadora, radar_init.cpp:9 */
+    double c;        /* propagation speed [m/s] System.VelocidadeLuz, radar_init.cpp:74 */
+    double fs;       /* receiver ADC rate [Hz]  RadarRx.fs, radar_init.cpp:44 (before upsampling) */
+    double n1, n2;   /* refractive indices      Environment, raw_data_processing.cpp:18-19 */
+    int upsample;    /* fast-time factor        factorInterp, proc_test_script.cpp:220 */
+    int reserved;
+} bsar_params;
+
+/* DEM as ProcessedDEM holds it (radar_params.h:75-79; loaded at
+ * proc_test_script.cpp:276-278): X_vec[nx], Y_vec[ny], Z_DEM[zrows][zcols]
+ * with Z[j][i] the height at (Y_vec[j], X_vec[i]).  Entries beyond
+ * zrows/zcols read as 0.0 (raw_data_processing.cpp:649-652). */
+typedef struct {
+    const double* x; int nx;
+    const double* y; int ny;
+    const double* z; int zrows; int zcols;
+} bsar_dem;
+
+typedef struct bsar_ctx bsar_ctx;
+typedef struct bsar_dataset bsar_dataset;
+
+/* Filled by bsar_backproject when non-NULL. */
+typedef struct {
+    double seconds;        /* device time of the back-projection launches (HIP events) */
+    double voxel_pulses;   /* voxels x pulses back-projected */
+    int64_t voxels;
+    int32_t pulses;
+    int32_t launches;      /* kernel launches used */
+    double r1_exact_frac;  /* MIXED: fraction of R1 means computed exactly (outside the table) */
+} bsar_report;
+
+/* ---- context ------------------------------------------------------------ */
+BSAR_API int  bsar_ctx_create(int device, bsar_ctx** out);
+BSAR_API void bsar_ctx_destroy(bsar_ctx* ctx);
+BSAR_API const char* bsar_last_error(const bsar_ctx* ctx);
+BSAR_API const char* bsar_status_string(int status);
+BSAR_API int  bsar_abi_version(void);
+BSAR_API int  bsar_ctx_synchronize(bsar_ctx* ctx);
+
+/* ---- stand-alone kernels ------------------------------------------------ */
+
+/* interpolateDEMElevation, raw_data_processing.cpp:615-680 (declared
+ * raw_data_processing.h:146), batched: z[k] = DEM(xy[2k], xy[2k+1]). */
+BSAR_API int bsar_dem_interp(bsar_ctx* ctx, const bsar_dem* dem, const double* xy, int64_t m, double* z);
+
+/* calculateRefractionPoint, raw_data_processing.cpp:455-572 (declared
+ * raw_data_processing.h:135-139), batched over m independent legs
+ * legs[k] = (x1,y1,z1,x2,y2,z2); out[k] = refraction point (x,y,z).  The
+ * argmin is bit-identical to the reference's FP64 grid search. */
+BSAR_API int bsar_refraction_points(bsar_ctx* ctx, const bsar_dem* dem, const double* legs, int64_t m,
+                           double n1, double n2, double* out);
+
+/* Refraction points of calculaSlantRangeFermat, raw_data_processing.cpp:112-
+ * 255, for V targets and the dataset's N pulses:
+ * ida[v][i] = refraction(Tx_i -> P_v, n1, n2), volta[v][i] = refraction(P_v -> Rx_i, n2, n1).
+ * pts is [V][3]; ida and volta are [V][N][3]. */
+BSAR_API int bsar_slant_range_fermat(bsar_ctx* ctx, const bsar_dataset* ds, const double* pts, int64_t V,
+                            double* ida, double* volta);
+
+/* splineInterpolation, interpolation.cpp:4-51 (declared interpolation.h:8),
+ * over N pulses at once as proc_test_script.cpp:326-350 drives it:
+ * in [N][S] complex (range fastest), out [N][S*factor] complex.  S >= 2. */
+BSAR_API int bsar_upsample(bsar_ctx* ctx, const double* in, int S, int N, int factor, double* out);
+
+/* FFTProcessor::fftCorrelation (conj = 1, pulse compression) and
+ * fftConvolution (conj = 0), fft_utils.cpp:63-137 (declared fft_utils.h:
+ * 35-60), applied to N columns of length S against one reference of length
+ * S, as processRawData drives them (raw_data_processing.cpp:406-446).
+ * sig and out are [N][S] complex (column = pulse, range fastest); S must be
+ * a power of two (the reference pads to one, fft_utils.cpp:71).
+ * precision: BSAR_FP64 runs complex128, BSAR_MIXED complex64 arithmetic. */
+BSAR_API int bsar_range_compress(bsar_ctx* ctx, const double* sig, int S, int N, const double* ref,
+                        int conj, int precision, double* out);
+
+/* createCHIRP, raw_data_processing.cpp:257-290: writes L = int(T*fs)+1
+ * samples when out != NULL; returns L (> 0) or a negative status. */
+BSAR_API int bsar_chirp(double f0, double f1, double T, double fs, double* out);
+
+/* ---- datasets (what proc_test_script.cpp:225-292 loads) ----------------- */
+
+/* DEM, Tx/Rx positions ([n_pulses][3], Tx_positions.csv / Rx_positions.csv)
+ * and constants.  The dataset keeps them resident on the context's GPU. */
+BSAR_API int bsar_dataset_create(bsar_ctx* ctx, const bsar_params* p, const bsar_dem* dem,
+                        const double* tx, const double* rx, int n_pulses, bsar_dataset** out);
+BSAR_API void bsar_dataset_destroy(bsar_dataset* ds);
+
+/* rootData_complex.bin payload: n_data pulses of S complex samples, range
+ * fastest (flat index r + p*S, proc_test_script.cpp:330).  Uploaded and
+ * upsampled by params.upsample on the GPU (proc_test_script.cpp:326-350);
+ * the upsampled block stays resident. */
+BSAR_API int bsar_dataset_load_rootdata(bsar_ctx* ctx, bsar_dataset* ds, const double* root, int S, int n_data);
+
+/* Same, when the samples are already upsampled ([n_data][S4] complex). */
+BSAR_API int bsar_dataset_load_upsampled(bsar_ctx* ctx, bsar_dataset* ds, const double* rcomp, int S4, int n_data);
+
+/* Device memory the dataset holds, in bytes. */
+BSAR_API int64_t bsar_dataset_device_bytes(const bsar_dataset* ds);
+
+/* ---- back-projection (proc_test_script.cpp:395-798) -------------------- */
+
+/* Image over the grid axes gx[nx], gy[ny], gz[nz] (grid_{x,y,z}Axis.csv).
+ * out is [nz][ny][nx] complex, x fastest -- the order export3DComplexMatrix
+ * writes (data_export.cpp:241-256); the reference's output[x][y][z] is
+ * out[(z*ny + y)*nx + x]. */
+BSAR_API int bsar_backproject(bsar_ctx* ctx, const bsar_dataset* ds,
+                     const double* gx, int nx, const double* gy, int ny, const double* gz, int nz,
+                     int precision, double* out, bsar_report* rep);
+
+/* Same, into device memory the caller owns (d_out: nx*ny*nz complex double
+ * on the context's device); no host transfer of the image. */
+BSAR_API int bsar_backproject_device(bsar_ctx* ctx, const bsar_dataset* ds,
+                            const double* gx, int nx, const double* gy, int ny, const double* gz, int nz,
+                            int precision, void* d_out, bsar_report* rep);
+
+/* ---- files (stage boundary build/output/proc/) ------------------------- */
+
+/* readBinary<double>, binary_reader.cpp:9-99: header (u32 ndims, u32
+ * isComplex, u32 typeLen, char[16], u64 dims[ndims]) + float64 payload.
+ * With buf == NULL only the header is parsed.  dims receives up to
+ * max_dims sizes; *count the element count; real files read as im = 0. */
+BSAR_API int bsar_read_matrix(const char* path, uint64_t* dims, int max_dims, int* ndims,
+                     int* is_complex, double* buf, uint64_t* count);
+
+/* export3DComplexMatrix, data_export.cpp:180-309: 3-D complex file (dims
+ * nx, ny, nz; data x fastest) plus <base>_grid_{x,y,z}.csv.  img is
+ * [nz][ny][nx] complex as bsar_backproject writes it. */
+BSAR_API int bsar_write_image(const char* path, const double* img, int nx, int ny, int nz,
+                     const double* gx, const double* gy, const double* gz);
+
+/* exportComplexMatrix, data_export.cpp:71-138: 2-D complex file,
+ * dims (rows, cols), data row-major. */
+BSAR_API int bsar_write_matrix2d(const char* path, const double* data, uint64_t rows, uint64_t cols);
+
+/* ---- accounting --------------------------------------------------------- */
+
+/* Algorithmic FP64 flop per voxel-pulse of the plain form: two 709-candidate
+ * refraction solves (46 flop per candidate + 64 set-up), R1 means (20 N),
+ * delay/phase, lerp and accumulate (48): 65,406 + 20 N. */
+BSAR_API double bsar_flops_per_voxel_pulse(int n_pulses);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BSAR_H */

@@ -1,0 +1,244 @@
+"""ctypes binding of liboracle.so (the CPU FP64 restatement) and of the
+compiled reference's shim (_ref/libref.so) when present.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker -- never by the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libref.so"
+REF_PROC = HERE / "_ref" / "proc_test_script"
+REF_TREE = Path("/root/reference")
+
+_dp = C.POINTER(C.c_double)
+
+
+class DemO(C.Structure):
+    _fields_ = [("x", _dp), ("nx", C.c_int), ("y", _dp), ("ny", C.c_int), ("z", _dp), ("zrows", C.c_int),
+                ("zcols", C.c_int)]
+
+
+class SimParams(C.Structure):
+    _fields_ = [("fc", C.c_double), ("c", C.c_double), ("fs", C.c_double), ("f0", C.c_double),
+                ("f1", C.c_double), ("T", C.c_double), ("fs_chirp", C.c_double), ("n1", C.c_double),
+                ("n2", C.c_double), ("S", C.c_int)]
+
+
+class BpParams(C.Structure):
+    _fields_ = [("c", C.c_double), ("fs4", C.c_double), ("lam", C.c_double), ("n1", C.c_double),
+                ("n2", C.c_double)]
+
+
+def build(ref: bool = False) -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    if ref and REF_TREE.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+
+
+_o = None
+
+
+def lib() -> C.CDLL:
+    global _o
+    if _o is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        L.or_dem_interp.restype = C.c_double
+        L.or_dem_interp.argtypes = [C.POINTER(DemO), C.c_double, C.c_double]
+        L.or_refraction_point.argtypes = [C.c_double] * 6 + [C.POINTER(DemO), C.c_double, C.c_double, _dp]
+        L.or_slant_range_fermat.argtypes = [C.POINTER(DemO), _dp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp,
+                                            _dp, _dp]
+        L.or_spline.argtypes = [_dp, C.c_size_t, C.c_int, _dp]
+        L.or_fft.argtypes = [_dp, C.c_size_t]
+        L.or_ifft.argtypes = [_dp, C.c_size_t]
+        L.or_fft_correlation.argtypes = [_dp, C.c_size_t, _dp, C.c_size_t, _dp]
+        L.or_fft_convolution.argtypes = [_dp, C.c_size_t, _dp, C.c_size_t, _dp]
+        L.or_chirp.restype = C.c_int
+        L.or_chirp.argtypes = [C.c_double] * 4 + [_dp]
+        L.or_simulate.argtypes = [C.POINTER(SimParams), C.POINTER(DemO), _dp, _dp, C.c_int, _dp, _dp, C.c_int,
+                                  _dp, _dp, _dp]
+        L.or_upsample.argtypes = [_dp, C.c_size_t, C.c_size_t, C.c_int, _dp]
+        L.or_backproject.argtypes = [C.POINTER(BpParams), C.POINTER(DemO), _dp, _dp, C.c_int, _dp, C.c_size_t,
+                                     C.c_size_t, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp,
+                                     C.POINTER(C.c_int64), C.c_int64, C.c_int, C.c_int]
+        L.or_flops_per_voxel_pulse.restype = C.c_double
+        L.or_flops_per_voxel_pulse.argtypes = [C.c_int]
+        _o = L
+    return _o
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+class _DemHold:
+    def __init__(self, x, y, z):
+        self.x, self.y, self.z = _f(x), _f(y), _f(z)
+        zr, zc = self.z.shape if self.z.ndim == 2 else (0, 0)
+        self.s = DemO(_p(self.x), len(self.x), _p(self.y), len(self.y), _p(self.z), zr, zc)
+
+
+def dem_interp(dem, xy):
+    d = _DemHold(*dem)
+    L = lib()
+    xy = _f(xy).reshape(-1, 2)
+    return np.array([L.or_dem_interp(C.byref(d.s), a, b) for a, b in xy])
+
+
+def refraction_points(dem, legs, n1, n2):
+    d = _DemHold(*dem)
+    L = lib()
+    legs = _f(legs).reshape(-1, 6)
+    out = np.empty((len(legs), 3))
+    q = np.empty(3)
+    for k, s in enumerate(legs):
+        L.or_refraction_point(*s, C.byref(d.s), n1, n2, _p(q))
+        out[k] = q
+    return out
+
+
+def slant_range_fermat(dem, tx, rx, P, n1, n2):
+    d = _DemHold(*dem)
+    tx, rx, P = _f(tx).reshape(-1, 3), _f(rx).reshape(-1, 3), _f(P)
+    n = len(tx)
+    ida, volta, r4 = np.empty((n, 3)), np.empty((n, 3)), np.empty((4, n))
+    lib().or_slant_range_fermat(C.byref(d.s), _p(tx), _p(rx), n, _p(P), n1, n2, _p(ida), _p(volta), _p(r4))
+    return ida, volta, r4
+
+
+def spline(data, factor=4):
+    d = _c(data)
+    out = np.empty(len(d) * factor, np.complex128)
+    lib().or_spline(_p(d.view(np.float64)), len(d), factor, _p(out.view(np.float64)))
+    return out
+
+
+def upsample(root, factor=4):
+    """root [N][S] complex (range fastest) -> [N][S*factor]."""
+    r = _c(root)
+    N, S = r.shape
+    out = np.empty((N, S * factor), np.complex128)
+    lib().or_upsample(_p(r.view(np.float64)), S, N, factor, _p(out.view(np.float64)))
+    return out
+
+
+def fft(a):
+    x = _c(a).copy()
+    lib().or_fft(_p(x.view(np.float64)), len(x))
+    return x
+
+
+def ifft(a):
+    x = _c(a).copy()
+    lib().or_ifft(_p(x.view(np.float64)), len(x))
+    return x
+
+
+def fft_correlation(sig, ref):
+    s, r = _c(sig), _c(ref)
+    out = np.empty(max(len(s), len(r)), np.complex128)
+    lib().or_fft_correlation(_p(s.view(np.float64)), len(s), _p(r.view(np.float64)), len(r),
+                             _p(out.view(np.float64)))
+    return out
+
+
+def fft_convolution(sig, ref):
+    s, r = _c(sig), _c(ref)
+    out = np.empty(max(len(s), len(r)), np.complex128)
+    lib().or_fft_convolution(_p(s.view(np.float64)), len(s), _p(r.view(np.float64)), len(r),
+                             _p(out.view(np.float64)))
+    return out
+
+
+def chirp(f0, f1, T, fs):
+    L = lib()
+    n = L.or_chirp(f0, f1, T, fs, None)
+    out = np.empty(n, np.complex128)
+    L.or_chirp(f0, f1, T, fs, _p(out.view(np.float64)))
+    return out
+
+
+def simulate(sim: dict, dem, tx, rx, scat, rcs):
+    """processRawData over scatterers: returns aux, raw, root as [S][N] complex."""
+    d = _DemHold(*dem)
+    tx, rx = _f(tx).reshape(-1, 3), _f(rx).reshape(-1, 3)
+    scat = _f(scat).reshape(-1, 3)
+    rcs = _c(rcs).reshape(-1)
+    n, S = len(tx), int(sim["S"])
+    sp = SimParams(sim["fc"], sim["c"], sim["fs"], sim["f0"], sim["f1"], sim["T"], sim["fs_chirp"], sim["n1"],
+                   sim["n2"], S)
+    aux = np.empty((S, n), np.complex128)
+    raw = np.empty((S, n), np.complex128)
+    root = np.empty((S, n), np.complex128)
+    lib().or_simulate(C.byref(sp), C.byref(d.s), _p(tx), _p(rx), n, _p(scat), _p(rcs.view(np.float64)), len(scat),
+                      _p(aux.view(np.float64)), _p(raw.view(np.float64)), _p(root.view(np.float64)))
+    return aux, raw, root
+
+
+def backproject(bp: dict, dem, tx, rx, rcomp, gx, gy, gz, subset=None, reflection=False, threads=0):
+    """Image [nz][ny][nx] complex (x fastest); voxels outside `subset` stay 0."""
+    d = _DemHold(*dem)
+    tx, rx = _f(tx).reshape(-1, 3), _f(rx).reshape(-1, 3)
+    rc = _c(rcomp)
+    ndata, S4 = rc.shape
+    gx, gy, gz = _f(gx), _f(gy), _f(gz)
+    out = np.zeros((len(gz), len(gy), len(gx)), np.complex128)
+    bpp = BpParams(bp["c"], bp["fs4"], bp["lam"], bp["n1"], bp["n2"])
+    if subset is None:
+        sub_p, nsub = None, 0
+    else:
+        sub = np.ascontiguousarray(subset, dtype=np.int64)
+        sub_p, nsub = sub.ctypes.data_as(C.POINTER(C.c_int64)), len(sub)
+    lib().or_backproject(C.byref(bpp), C.byref(d.s), _p(tx), _p(rx), len(tx), _p(rc.view(np.float64)), S4, ndata,
+                         _p(gx), len(gx), _p(gy), len(gy), _p(gz), len(gz), _p(out.view(np.float64)), sub_p, nsub,
+                         int(reflection), int(threads))
+    return out
+
+
+def flops_per_voxel_pulse(n):
+    return lib().or_flops_per_voxel_pulse(n)
+
+
+# ---- compiled reference (present only where it was built from /root/reference)
+
+_r = None
+
+
+def ref_lib():
+    global _r
+    if _r is None:
+        if not REF_LIB.exists():
+            return None
+        L = C.CDLL(str(REF_LIB))
+        dem_args = [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int]
+        L.ref_dem_interp.argtypes = dem_args + [_dp, C.c_int, _dp]
+        L.ref_refraction.argtypes = dem_args + [_dp, C.c_int, C.c_double, C.c_double, _dp]
+        L.ref_slant_range.argtypes = dem_args + [_dp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.ref_spline.argtypes = [_dp, C.c_int, C.c_int, _dp]
+        L.ref_fft.argtypes = [_dp, C.c_int]
+        L.ref_ifft.argtypes = [_dp, C.c_int]
+        L.ref_fft_correlation.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp]
+        L.ref_fft_convolution.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp]
+        L.ref_chirp.restype = C.c_int
+        L.ref_chirp.argtypes = [C.c_double] * 4 + [_dp]
+        L.ref_process_raw.argtypes = dem_args + [_dp, _dp, C.c_int, _dp, C.c_double] + [C.c_double] * 9 + \
+            [C.c_int, _dp, _dp, _dp]
+        _r = L
+    return _r

@@ -1,0 +1,141 @@
+"""HIP kernels against the oracle and the reference's golden vectors.
+
+Bit-exact for the DEM lookup, the refraction search, the slant-range
+refraction points and the spline upsampling (integer/index work and pure
++ - * / sqrt in the reference's order).  Range compression runs a different
+(exact-twiddle) FFT: FP64 within 1e-12 of the reference's output relative to
+its peak, FP32 within 2e-5."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from test_oracle_golden import DEMS, bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", DEMS)
+def test_dem_interp_bit_exact(bsar, ctx, golden, name):
+    g = golden
+    dem = bsar.Dem(g[f"dem_{name}_x"], g[f"dem_{name}_y"], g[f"dem_{name}_z"])
+    out = ctx.dem_interp(dem, g[f"dem_{name}_xy"])
+    assert bits_equal(out, g[f"dem_{name}_out"])
+
+
+@pytest.mark.parametrize("name", DEMS)
+@pytest.mark.parametrize("tag", ["a", "b", "eq"])
+def test_refraction_bit_exact(bsar, ctx, golden, name, tag):
+    g = golden
+    dem = bsar.Dem(g[f"dem_{name}_x"], g[f"dem_{name}_y"], g[f"dem_{name}_z"])
+    n1, n2 = g[f"refr_{name}_{tag}_n"]
+    out = ctx.refraction_points(dem, g[f"refr_{name}_{tag}_legs"], n1, n2)
+    assert bits_equal(out, g[f"refr_{name}_{tag}_out"])
+
+
+def test_refraction_empty_dem(bsar, ctx, golden):
+    e = np.zeros(0)
+    out = ctx.refraction_points(bsar.Dem(e, e, np.zeros((0, 0))), golden["refr_empty_legs"], 1.0, 2.0)
+    assert bits_equal(out, golden["refr_empty_out"])
+
+
+def test_refraction_random_legs_vs_oracle(bsar, ctx, oracle, scene_mod):
+    rng = np.random.default_rng(42)
+    dem = scene_mod.dem_bumps(rng)
+    M = 4000
+    top = np.stack([rng.uniform(-20, 20, M), rng.uniform(-20, 20, M), rng.uniform(10, 14, M)], 1)
+    bot = np.stack([rng.uniform(19, 21, M), rng.uniform(-1, 1, M), rng.uniform(-6, -4, M)], 1)
+    legs = np.concatenate([top, bot], 1)
+    legs[M // 2:] = np.concatenate([bot, top], 1)[M // 2:]
+    gpu = ctx.refraction_points(bsar.Dem(*dem), legs, 1.0, 2.449489742783178)
+    ref = oracle.refraction_points(dem, legs, 1.0, 2.449489742783178)
+    assert bits_equal(gpu, ref)
+
+
+def test_rough_dem_refraction_vs_oracle(bsar, ctx, oracle, scene_mod):
+    """167 x 100 fBm terrain (read through the cache, not LDS)."""
+    rng = np.random.default_rng(1003)
+    dem = scene_mod.dem_fbm(rng)
+    M = 1500
+    top = np.stack([rng.uniform(-15, 15, M), rng.uniform(-15, 15, M), rng.uniform(40, 60, M)], 1)
+    bot = np.stack([rng.uniform(-2, 2, M), rng.uniform(-2, 2, M), rng.uniform(-30, -10, M)], 1)
+    legs = np.concatenate([top, bot], 1)
+    n2 = float(np.sqrt(rng.uniform(4, 9)))
+    gpu = ctx.refraction_points(bsar.Dem(*dem), legs, 1.0, n2)
+    ref = oracle.refraction_points(dem, legs, 1.0, n2)
+    assert bits_equal(gpu, ref)
+
+
+def test_slant_range_bit_exact(bsar, ctx, golden):
+    g = golden
+    p = bsar.Params(1e9, 3e8, 800e6, 1.0, 2.449489742783178)
+    ds = ctx.dataset(p, bsar.Dem(g["dem_bumps_x"], g["dem_bumps_y"], g["dem_bumps_z"]), g["sr_tx"], g["sr_rx"])
+    ida, volta = ds.slant_range_fermat(g["sr_P"][None, :])
+    assert bits_equal(ida[0], g["sr_ida"])
+    assert bits_equal(volta[0], g["sr_volta"])
+
+
+@pytest.mark.parametrize("nf", [(2, 4), (3, 4), (5, 3), (64, 4), (257, 4), (16, 1), (9, 7)])
+def test_spline_bit_exact(ctx, golden, nf):
+    n, f = nf
+    d = golden[f"spline_{n}_{f}_in"].view(np.complex128)
+    out = ctx.upsample(d[None, :], f)
+    assert bits_equal(out[0].view(np.float64), golden[f"spline_{n}_{f}_out"])
+
+
+def test_upsample_batch_vs_oracle(ctx, oracle):
+    rng = np.random.default_rng(5)
+    root = rng.normal(size=(37, 1000)) + 1j * rng.normal(size=(37, 1000))
+    assert bits_equal(ctx.upsample(root, 4).view(np.float64), oracle.upsample(root, 4).view(np.float64))
+
+
+@pytest.mark.parametrize("S", [64, 1024, 8192, 16384])
+@pytest.mark.parametrize("conj", [True, False])
+def test_range_compress_fp64(ctx, oracle, S, conj):
+    rng = np.random.default_rng(S + conj)
+    N = 6
+    sig = rng.normal(size=(N, S)) + 1j * rng.normal(size=(N, S))
+    ref = rng.normal(size=S) + 1j * rng.normal(size=S)
+    gpu = ctx.range_compress(sig, ref, conj=conj)
+    f = oracle.fft_correlation if conj else oracle.fft_convolution
+    want = np.stack([f(s, ref) for s in sig])
+    assert np.abs(gpu - want).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("S", [4096, 16384, 32768])
+def test_range_compress_fp32(bsar, ctx, oracle, S):
+    rng = np.random.default_rng(S)
+    N = 4
+    sig = rng.normal(size=(N, S)) + 1j * rng.normal(size=(N, S))
+    ref = rng.normal(size=S) + 1j * rng.normal(size=S)
+    gpu = ctx.range_compress(sig, ref, conj=True, precision=bsar.MIXED)
+    want = np.stack([oracle.fft_correlation(s, ref) for s in sig])
+    assert np.abs(gpu - want).max() <= 2e-5 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("n", [7, 12])
+def test_range_compress_non_power_of_two(ctx, golden, n):
+    """The reference pads to the next power of two and keeps the first n."""
+    a = golden[f"fft_{n}_in"].view(np.complex128)
+    b = golden[f"corr_{n}_ref"].view(np.complex128)
+    gpu = ctx.range_compress(a[None, :], b, conj=True)[0]
+    want = golden[f"corr_{n}_out"].view(np.complex128)
+    assert np.abs(gpu - want).max() <= 1e-12 * np.abs(want).max()
+    gpu = ctx.range_compress(a[None, :], b, conj=False)[0]
+    want = golden[f"conv_{n}_out"].view(np.complex128)
+    assert np.abs(gpu - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_pulse_compression_of_reference_simulation(bsar, ctx, golden):
+    """processRawData's last step: rootData = corr(rawData, reference)."""
+    g = golden
+    fc, c, fs_rx, fs_tx, f0, f1, T, n1, n2 = g["sim_args"]
+    S = int(g["sim_S"][0])
+    raw = (g["sim_raw"][..., 0] + 1j * g["sim_raw"][..., 1]).T        # [pulse][range]
+    root = (g["sim_root"][..., 0] + 1j * g["sim_root"][..., 1]).T
+    chirp = bsar.chirp(f0, f1, T, fs_tx)
+    L = len(chirp)
+    ref = np.zeros(S, complex)
+    ref[: L - L // 2] = chirp[L // 2:]
+    ref[S - L // 2:] = chirp[: L // 2]
+    gpu = ctx.range_compress(raw, ref, conj=True)
+    assert np.abs(gpu - root).max() <= 1e-12 * np.abs(root).max()
