@@ -1,0 +1,238 @@
+"""Benchmark: voxel-pulse back-projections per second on one MI355X, mixed
+precision vs plain FP64, on the 201x201x101-voxel x 2048-pulse image
+(8192 fast-time samples, upsampled x4 to 32768), with the error of the mixed
+image against the FP64 image in dB and the reference's OpenMP CPU path timed
+on the same host.
+
+    python bench.py [--steps K] [--warmup W] [--workload large|small|...]
+
+One step = one full mixed-precision image of the workload with the
+range-compressed data already resident in HBM.  Prints one JSON line on
+stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import re
+import shutil
+import struct
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+PKG_DIR = ROOT / "c-_bistatic_sar_amd"
+
+P64 = 78.6e12     # MI355X FP64 vector peak, FLOP/s
+P32 = 157.3e12    # MI355X FP32 vector peak, FLOP/s
+
+# Algorithmic flop per voxel-pulse (add/sub/mul/div/sqrt = 1, FMA = 2,
+# sin = cos = 1; loop-invariant terms hoisted):
+#   plain FP64 form: 65,406 + 20 N (two 709-candidate refraction solves at 46
+#   flop each + 64 set-up, R1 means 2 (10 N + 1), 48 flop of delay/phase/lerp/
+#   accumulate);
+#   mixed form: the same two FP64 solves (65,356) + FP64 R2 (18) + two
+#   tricubic table lookups (2 x 163: 3 x 10 weights, 64 x 2 + 5 x 2 + ...)
+#   + FP64 delay/phase/reduction (14) in FP64; lerp, sin/cos, complex
+#   product and two Kahan adds (27) in FP32.
+def flops_fp64(n: int) -> tuple[float, float]:
+    return 65406.0 + 20.0 * n, 0.0
+
+
+def flops_mixed(n: int) -> tuple[float, float]:
+    return 65356.0 + 18.0 + 2 * 163.0 + 14.0, 27.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def load(name: str, path: Path, pkg: bool = False):
+    if name in sys.modules:
+        return sys.modules[name]
+    kw = dict(submodule_search_locations=[str(path.parent)]) if pkg else {}
+    spec = importlib.util.spec_from_file_location(name, path, **kw)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def roofline(rep: dict, n: int, form: str, traffic) -> dict:
+    f64, f32 = (flops_mixed if form == "mixed" else flops_fp64)(n)
+    launches = max(int(rep["launches"]) - 1, 1) if rep["launches"] > 1 else 1
+    t_launch = rep["seconds"] / launches
+    vp_launch = rep["voxel_pulses"] / launches
+    eq = (f64 + f32 * P64 / P32) * vp_launch          # FP64-equivalent flop per launch
+    ach = eq / t_launch
+    return {"bound": "valu", "achieved": round(ach / 1e12, 3), "peak": P64 / 1e12, "unit": "TFLOP/s",
+            "frac": round(ach / P64, 4), "traffic": traffic,
+            "kernel": "k_bp_mixed" if form == "mixed" else "k_bp_fp64",
+            "avg_launch_ms": round(t_launch * 1e3, 3), "launches": launches,
+            "flop_per_voxel_pulse": {"fp64": f64, "fp32": f32}}
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    p = ROOT / "profiles" / "traffic_bp_large.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return d.get(kernel, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def write_reference_inputs(proc: Path, sc, root: np.ndarray, gx, gy, gz) -> None:
+    """build/output/proc/ file set in the reference's formats (Appendix of
+    data_export.cpp: binary matrix header + CSVs written %.6f)."""
+    proc.mkdir(parents=True, exist_ok=True)
+    N, S = root.shape
+    with open(proc / "rootData_complex.bin", "wb") as f:
+        f.write(struct.pack("<III", 2, 1, 6) + b"double".ljust(16, b"\0") + struct.pack("<QQ", S, N))
+        f.write(np.ascontiguousarray(root).tobytes())
+
+    def vec(name, v):
+        (proc / name).write_text(",".join("%.6f" % x for x in v) + "\n")
+
+    def mat(name, m):
+        (proc / name).write_text("".join(",".join("%.6f" % x for x in row) + "\n" for row in m))
+
+    vec("grid_xAxis.csv", gx)
+    vec("grid_yAxis.csv", gy)
+    vec("grid_zAxis.csv", gz)
+    vec("DEM_X_vec.csv", sc.dem[0])
+    vec("DEM_Y_vec.csv", sc.dem[1])
+    mat("DEM_Z_matrix.csv", sc.dem[2])
+    mat("Tx_positions.csv", sc.tx)
+    mat("Rx_positions.csv", sc.rx)
+    (proc / "environment.txt").write_text("n1=%r\nn2=%r\n" % (sc.n1, sc.n2))
+
+
+def cpu_baseline(sc, root, target_s: float = 20.0) -> dict | None:
+    """The reference's own OpenMP back-projection (oracle/_ref/proc_test_script,
+    built from the reference sources) on a strided voxel subset of the same
+    workload; falls back to the C restatement (oracle/liboracle.so)."""
+    ref = ROOT / "oracle" / "_ref" / "proc_test_script"
+    nx, ny, nz = 4, 4, 8
+    sub = lambda a, m: a[np.linspace(0, len(a) - 1, m).round().astype(int)]
+    gx, gy, gz = sub(sc.gx, nx), sub(sc.gy, ny), sub(sc.gz, nz)
+    vox = nx * ny * nz
+    cores = os.cpu_count() or 1
+    if ref.exists():
+        work = Path(tempfile.mkdtemp(prefix="bsar_cpu_"))
+        try:
+            write_reference_inputs(work / "build" / "output" / "proc", sc, root, gx, gy, gz)
+            threads = min(int(os.environ.get("OMP_NUM_THREADS", cores)), cores)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            t0 = time.time()
+            r = subprocess.run([str(ref)], cwd=work, capture_output=True, text=True, env=env, timeout=900)
+            wall = time.time() - t0
+            if r.returncode == 0:
+                m = re.search(r"Procesamiento paralelo completado en: (\d+) ms", r.stdout)
+                if m:
+                    secs = int(m.group(1)) / 1e3
+                    used = min(threads, 12)   # proc_test_script.cpp:428 caps the team at 12
+                    return {"value": round(vox * sc.n / secs, 1), "unit": "voxel-pulse/s", "cores": used,
+                            "kind": "reference",
+                            "sample": f"{vox} voxels ({nx}x{ny}x{nz} strided subset) x {sc.n} pulses; "
+                                      f"voxel loop {secs:.1f} s (wall {wall:.1f} s with file load + x4 spline); "
+                                      "reference constants fc=10 GHz, fs=60 MHz x4 compiled in, same work per "
+                                      "voxel-pulse; logging to a pipe"}
+            log(f"reference CPU run failed (rc={r.returncode}); using the C restatement")
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    up = O.upsample(root[:, :], sc.upsample)
+    ids = np.arange(vox)
+    t0 = time.time()
+    O.backproject(sc.bp_params(), sc.dem, sc.tx, sc.rx, up, gx, gy, gz, subset=ids, reflection=True, threads=cores)
+    secs = time.time() - t0
+    return {"value": round(vox * sc.n / secs, 1), "unit": "voxel-pulse/s", "cores": cores, "kind": "port",
+            "sample": f"{vox} voxels x {sc.n} pulses, C restatement with the reflection search"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--workload", default="large")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "the image shards by voxel slab across GPUs with no exchange, see the design notes.", file=sys.stderr)
+        return 2
+
+    pkg = load("bsar_amd", PKG_DIR / "__init__.py", pkg=True)
+    scene = load("bsar_amd.scene", PKG_DIR / "scene.py")
+    sc = scene.make_scene(args.workload)
+    V, N = sc.voxels, sc.n
+    log(f"workload {sc.name}: {len(sc.gx)}x{len(sc.gy)}x{len(sc.gz)} voxels x {N} pulses, {sc.S} samples")
+    with pkg.Context(0) as ctx:
+        t0 = time.time()
+        root = scene.simulate(ctx, sc)
+        log(f"forward model (GPU Fermat + FFT) {time.time() - t0:.1f} s")
+        ds = ctx.dataset(pkg.Params(sc.fc, sc.c, sc.fs, sc.n1, sc.n2, sc.upsample), pkg.Dem(*sc.dem), sc.tx, sc.rx)
+        ds.load_rootdata(root)
+        for w in range(args.warmup):
+            _, rep = ds.backproject(sc.gx, sc.gy, sc.gz, pkg.MIXED)
+            log(f"warmup {w + 1}/{args.warmup}: {rep['seconds']:.2f} s")
+        lat, reps = [], []
+        pkg.lib().bsar_ctx_synchronize(ctx.h)
+        t_start = time.perf_counter()
+        for k in range(args.steps):
+            t1 = time.perf_counter()
+            img_m, rep = ds.backproject(sc.gx, sc.gy, sc.gz, pkg.MIXED)
+            lat.append(time.perf_counter() - t1)
+            reps.append(rep)
+            log(f"step {k + 1}/{args.steps}: {lat[-1]:.2f} s wall, {rep['seconds']:.2f} s device, "
+                f"{rep['launches']} launches, R1 exact fraction {rep['r1_exact_frac']:.2e}")
+        pkg.lib().bsar_ctx_synchronize(ctx.h)
+        elapsed = time.perf_counter() - t_start
+        value = V * N * args.steps / elapsed
+        out = {"metric": "voxel-pulse BP/s, mixed precision (FP64 refraction + delay, FP32 interp/Kahan accum)",
+               "value": round(value, 1), "unit": "voxel-pulse/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+               "data": "synthetic (seeded scene, GPU forward model, range-compressed on device)",
+               "config": {"workload": f"{sc.name}: {len(sc.gx)}x{len(sc.gy)}x{len(sc.gz)} voxels x {N} pulses, "
+                                      f"{sc.S} samples x{sc.upsample}", "voxels": V, "pulses": N,
+                          "samples": sc.S, "precision": "mixed"},
+               "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 1),
+                              "p99": round(float(np.percentile(lat, 99)) * 1e3, 1)}}
+        out["roofline"] = roofline(reps[-1], N, "mixed", traffic_from_profiles("k_bp_mixed"))
+        if not args.no_fp64:
+            t1 = time.perf_counter()
+            img64, rep64 = ds.backproject(sc.gx, sc.gy, sc.gz, pkg.FP64)
+            w64 = time.perf_counter() - t1
+            log(f"fp64 image: {w64:.2f} s wall, {rep64['seconds']:.2f} s device")
+            err = float(np.abs(img_m - img64).max() / np.abs(img64).max())
+            out["fp64"] = {"value": round(V * N / w64, 1), "ms": round(w64 * 1e3, 1),
+                           "roofline": roofline(rep64, N, "fp64", traffic_from_profiles("k_bp_fp64"))}
+            out["mixed_vs_fp64"] = {"speedup": round(w64 / (elapsed / args.steps), 3),
+                                    "peak_err_db": round(20 * np.log10(max(err, 1e-300)), 2)}
+        ds.close()
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cb = cpu_baseline(sc, root)
+        out["cpu_baseline"] = cb
+        if cb:
+            out["vs_cpu"] = round(value / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

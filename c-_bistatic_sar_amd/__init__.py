@@ -9,8 +9,10 @@ The directory name is not a Python identifier; load it with
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -84,6 +86,17 @@ SIGNATURES = {
 }
 
 _lib = None
+_live = weakref.WeakSet()     # open contexts and datasets, closed before interpreter exit
+
+
+@atexit.register
+def _close_all():
+    objs = list(_live)
+    for o in objs:                       # datasets first, then their contexts
+        if isinstance(o, Dataset):
+            o.close()
+    for o in objs:
+        o.close()
 
 
 def lib() -> C.CDLL:
@@ -151,9 +164,13 @@ class Context:
         if st != 0:
             raise BsarError(st, f"bsar_ctx_create(device={device})")
         self.h = h
+        self._children = weakref.WeakSet()
+        _live.add(self)
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
+            for d in list(self._children):
+                d.close()
             self.L.bsar_ctx_destroy(self.h)
             self.h = None
 
@@ -232,6 +249,8 @@ class Dataset:
         ctx._chk(self.L.bsar_dataset_create(ctx.h, C.byref(p), C.byref(d), _ptr(self.tx), _ptr(self.rx), self.n,
                                             C.byref(h)), "dataset_create")
         self.h = h
+        ctx._children.add(self)
+        _live.add(self)
 
     def close(self):
         if getattr(self, "h", None):

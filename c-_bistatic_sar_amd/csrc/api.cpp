@@ -98,6 +98,12 @@ BpPlan plan_bp(int64_t nvox, int n_pulses, double cost_per_vp)
     BpPlan p;
     p.nvox = nvox;
     p.nvox_pad = (nvox + 63) / 64 * 64;
+    if (p.nvox_pad <= 0) {            // empty grid: nothing is launched
+        p.nvox_pad = 64;
+        p.glen = std::max(n_pulses, 1);
+        p.wlen = p.glen;
+        return p;
+    }
     const int64_t target_threads = int64_t(1) << 19;
     int64_t groups = (target_threads + p.nvox_pad - 1) / p.nvox_pad;
     groups = std::max<int64_t>(1, std::min<int64_t>(groups, std::max(n_pulses, 1)));
@@ -427,8 +433,10 @@ int bsar_dataset_create(bsar_ctx* c, const bsar_params* p, const bsar_dem* dem, 
 void bsar_dataset_destroy(bsar_dataset* ds)
 {
     if (!ds) return;
+    // The dataset may outlive the context that created it: wait on the device,
+    // never on the creator's stream.
     (void)hipSetDevice(ds->device);
-    if (ds->owner && ds->owner->stream) (void)hipStreamSynchronize(ds->owner->stream);
+    (void)hipDeviceSynchronize();
     bsar_r1tab_free(ds);
     if (ds->dX) (void)hipFree(ds->dX);
     if (ds->dY) (void)hipFree(ds->dY);

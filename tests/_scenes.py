@@ -29,14 +29,14 @@ def bumps_dem(n=12, spacing=30.0, seed=3):
 class Tiny:
     """Handful of voxels and pulses; every check against the oracle is cheap."""
 
-    def __init__(self, n=24, S=512, grid=(7, 5, 4), seed=9, fc=1e9, fs=800e6, dem=None):
+    def __init__(self, n=24, S=512, grid=(7, 5, 4), seed=9, fc=1e9, fs=800e6, dem=None, z0=0.0):
         self.rng = np.random.default_rng(seed)
         self.n, self.S = n, S
-        self.tx, self.rx = spiral(n), spiral(n, np.pi / 2)
+        self.tx, self.rx = spiral(n, h=z0 + 12.0), spiral(n, np.pi / 2, h=z0 + 12.0)
         self.dem = bumps_dem() if dem is None else dem
         self.gx = q6(19.85 + 0.07 * np.arange(grid[0]))
         self.gy = q6(-0.15 + 0.07 * np.arange(grid[1]))
-        self.gz = q6(-5.2 + 0.13 * np.arange(grid[2]))
+        self.gz = q6(z0 - 5.2 + 0.13 * np.arange(grid[2]))
         self.fc, self.c, self.fs, self.n1, self.n2, self.up = fc, 3e8, fs, 1.0, N2, 4
 
     def params(self, bsar):
@@ -59,7 +59,10 @@ class Tiny:
 
 
 def peak_err(a, ref):
-    return float(np.abs(a - ref).max() / np.abs(ref).max())
+    """max |a - ref| / max |ref| (absolute max |a - ref| when ref is all zero)."""
+    d = float(np.abs(a - ref).max()) if np.size(ref) else 0.0
+    m = float(np.abs(ref).max()) if np.size(ref) else 0.0
+    return d / m if m > 0 else d
 
 
 def db(x):

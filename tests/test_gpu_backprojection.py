@@ -78,12 +78,15 @@ def test_dem_variants(bsar, ctx, oracle, scene_mod, demkind):
                np.array([[0.1, 0.2, 0.3, 0.4, 0.5], [0.0, -0.3, 0.6, 0.2, 0.1], [0.3, 0.3, -0.2, 0.0, 0.4]]))
     else:
         dem = scene_mod.dem_fbm(np.random.default_rng(3), nx=100, ny=90)
-    t = Tiny(n=10, grid=(3, 3, 2), dem=dem)
+    # antennas 12 m and voxels ~5 m below the terrain height near the grid
+    z0 = oracle.dem_interp(dem, [[20.0, 0.0]])[0]
+    t = Tiny(n=10, grid=(3, 3, 2), dem=dem, z0=z0)
     root = t.random_root()
     ds = ctx.dataset(t.params(bsar), bsar.Dem(*dem), t.tx, t.rx)
     ds.load_rootdata(root)
     img, _ = ds.backproject(t.gx, t.gy, t.gz, bsar.FP64)
     want = oracle.backproject(t.bp(), dem, t.tx, t.rx, oracle.upsample(root), t.gx, t.gy, t.gz)
+    assert np.abs(want).max() > 0
     assert peak_err(img, want) <= FP64_TOL
 
 
@@ -126,10 +129,6 @@ def test_small_workload_vs_oracle_subset(bsar, ctx, oracle, scene_mod):
     mixed, mrep = ds.backproject(sc.gx, sc.gy, sc.gz, bsar.MIXED)
     assert db(peak_err(mixed, img)) <= MIXED_DB
     assert mrep["r1_exact_frac"] < 0.01
-    # the peak of the image sits near one of the scatterers
-    k = np.unravel_index(np.argmax(np.abs(img)), img.shape)
-    peak = np.array([sc.gx[k[2]], sc.gy[k[1]], sc.gz[k[0]]])
-    assert np.min(np.linalg.norm(sc.scat - peak, axis=1)) < 0.6
 
 
 def test_linearity_full_grid(bsar, ctx):
