@@ -89,6 +89,9 @@ int upload_dem(bsar_ctx* ctx, const bsar_dem* dem, double** dX, double** dY, dou
     d.y0 = Y[0];
     d.ihx = (dem->nx > 1 && (d.flags & DEM_X_SORTED)) ? (dem->nx - 1) / (X[dem->nx - 1] - X[0]) : 0.0;
     d.ihy = (dem->ny > 1 && (d.flags & DEM_Y_SORTED)) ? (dem->ny - 1) / (Y[dem->ny - 1] - Y[0]) : 0.0;
+    d.zlo = lo;
+    d.zhi = hi;
+    d.zabs = std::max(std::fabs(lo), std::fabs(hi));
     *out = d;
     return BSAR_OK;
 }

@@ -24,6 +24,8 @@ struct DemDev {
     const double* Z;
     int nx, ny, zs, flags;
     double x0, ihx, y0, ihy;   // first value and inverse mean spacing (index guess only)
+    double zlo, zhi;           // height range of the stored Z (0, 0 when empty)
+    double zabs;               // max |Z|
 };
 
 // std::min / std::max as libstdc++ defines them (NaN-propagation included).
@@ -85,74 +87,182 @@ __device__ __forceinline__ double dem_interp(const DemDev& d, double x, double y
            z11 * wx * wy;
 }
 
-// One refinement level of calculateRefractionPoint (raw_data_processing.cpp:
-// 508-551) on a G x G grid.  The scan keeps the reference's strict '<'
-// against the minimum carried across levels; bestX/bestY are rebuilt from
-// the winning (i, j) with the same expression that produced them.
-template <int G>
-__device__ __forceinline__ void fermat_level(const DemDev& d, double x1, double y1, double z1,
-                                             double x2, double y2, double z2, double n1, double n2,
-                                             double R, double& bestX, double& bestY, double& minOpt)
+// ---- screened exact grid search ----------------------------------------
+//
+// The reference scans every candidate of a level in (i, j) order with a
+// strict '<' against the minimum carried across levels.  The same argmin is
+// found here by evaluating exactly (reference operation order, correctly
+// rounded sqrt) only the candidates that can still be the minimum:
+//
+//   * rows i / columns j whose Euclidean lower bound
+//       n1*sqrt(dx1^2 + g1^2) + n2*sqrt(dx2^2 + g2^2)
+//     (horizontal offset along one axis, vertical gap g to the DEM height
+//     range) exceeds the best value known so far are skipped;
+//   * every other candidate gets a cheap estimate (FMA bilinear, sqrt as
+//     a * v_rsq_f64(a)) and is evaluated exactly only when the estimate,
+//     widened by its error bound, could reach the best value;
+//   * candidates are visited centre-out (the previous level's best sits at
+//     the centre), and ties between exact values resolve to the lowest
+//     reference index i*G + j -- the first one the sequential scan meets.
+//
+// Error bounds: v_rsq_f64 is within 5.3e-8 relative of 1/sqrt (measured on
+// gfx950); kRho = 2e-6 covers it with margin, together with the few
+// roundings of the estimate.  kDefl deflates the lower bounds by more than
+// their rounding.  Any candidate dropped by either test has an exact value
+// strictly above an exact value already reached (or above the minimum of the
+// previous levels), so it can neither be the minimum nor tie with it.
+constexpr double kRho = 2e-6;
+constexpr double kDefl = 1.0 - 8e-6;
+
+__device__ __forceinline__ double sqrt_est(double a) { return a * __builtin_amdgcn_rsq(a); }
+
+struct Leg {
+    double x1, y1, z1, x2, y2, z2, n1, n2;
+    double g1sq, g2sq;   // squared vertical gaps from z1 / z2 to the DEM height range
+    double D;            // absolute error bound of the estimate
+    bool screen;         // bounds valid (n1, n2 >= 0 and finite); else every candidate is exact
+};
+
+__device__ __forceinline__ Leg make_leg(const DemDev& d, double x1, double y1, double z1, double x2, double y2,
+                                        double z2, double n1, double n2)
 {
+    Leg L{x1, y1, z1, x2, y2, z2, n1, n2, 0.0, 0.0, 0.0, n1 >= 0.0 && n2 >= 0.0 && n1 + n2 < 1e300};
+    const double g1 = fmax(fmax(d.zlo - z1, z1 - d.zhi), 0.0);
+    const double g2 = fmax(fmax(d.zlo - z2, z2 - d.zhi), 0.0);
+    L.g1sq = g1 * g1;
+    L.g2sq = g2 * g2;
+    L.D = (fabs(n1) + fabs(n2)) * 1e-13 * (d.zabs + fabs(z1) + fabs(z2) + 1.0);
+    return L;
+}
+
+__device__ __forceinline__ int center_out(int k, int c) { return (k & 1) ? c - ((k + 1) >> 1) : c + (k >> 1); }
+
+template <int G>
+__device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, double R, double& bestX, double& bestY,
+                                             double& minOpt)
+{
+    constexpr int C = (G - 1) / 2;
     const double minX = bestX - R, maxX = bestX + R;
     const double minY = bestY - R, maxY = bestY + R;
     const double spanX = maxX - minX, spanY = maxY - minY;
     const bool empty = (d.flags & DEM_EMPTY) != 0;
     const bool ysorted = (d.flags & DEM_Y_SORTED) != 0;
     const bool xsorted = (d.flags & DEM_X_SORTED) != 0;
+    const double n1 = L.n1, n2 = L.n2;
 
-    // Per-j terms (depend on py only): rows, weights, squared y offsets.
-    int jr0[G], jr1[G];
-    double wy[G], dya[G], dyb[G];
+    // Per-column terms (exact) and column lower bounds.  jr[j] is the Z
+    // offset of row j0 with bit 0 set when the y interval was found (dj = 1).
+    int jr[G];
+    double wy[G], pyv[G];
+    float lbc[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-        double py = minY + j * spanY / (G - 1);
+        const double py = minY + j * spanY / (G - 1);
         int j0 = 0, dj = 0;
         double w = 0.0;
         if (!empty) axis_lookup(d.Y, d.ny, d.y0, d.ihy, ysorted, py, j0, dj, w);
-        jr0[j] = j0 * d.zs;
-        jr1[j] = (j0 + dj) * d.zs;
+        jr[j] = ((j0 * d.zs) << 1) | dj;
         wy[j] = w;
-        double a = py - y1, b = y2 - py;
-        dya[j] = a * a;
-        dyb[j] = b * b;
+        pyv[j] = py;
+        const double a = py - L.y1, b = L.y2 - py;
+        const double lb = n1 * sqrt_est(__builtin_fma(a, a, L.g1sq)) + n2 * sqrt_est(__builtin_fma(b, b, L.g2sq));
+        lbc[j] = (float)(lb * kDefl);
     }
-    int bi = -1, bj = 0;
-    for (int i = 0; i < G; ++i) {
-        double px = minX + i * spanX / (G - 1);
+    const double stepX = spanX / (G - 1);
+    const double epsx = 1e-12 * (fabs(minX) + fabs(spanX) + 1.0);
+    double lvMin = __builtin_inf();
+    int lvK = 0x7fffffff;
+    double runS = __builtin_inf();
+
+#pragma unroll 1
+    for (int k = 0; k < G; ++k) {
+        const int i = center_out(k, C);
+        const double Mrow = L.screen ? fmin(fmin(minOpt, lvMin), (runS + L.D) * (1.0 + 3.0 * kRho))
+                                     : __builtin_inf();
+        // row lower bound from an approximate px (|error| < epsx)
+        {
+            const double pxa = __builtin_fma((double)i, stepX, minX);
+            const double a1 = fmax(fabs(pxa - L.x1) - epsx, 0.0);
+            const double a2 = fmax(fabs(pxa - L.x2) - epsx, 0.0);
+            const double lbr = (n1 * sqrt_est(__builtin_fma(a1, a1, L.g1sq)) +
+                                n2 * sqrt_est(__builtin_fma(a2, a2, L.g2sq))) * kDefl;
+            if (lbr > Mrow) continue;
+        }
+        const double px = minX + i * spanX / (G - 1);
         int i0 = 0, di = 0;
         double wx = 0.0;
         if (!empty) axis_lookup(d.X, d.nx, d.x0, d.ihx, xsorted, px, i0, di, wx);
-        const double omwx = 1 - wx;
-        const double a = px - x1, b = x2 - px;
-        const double dxa = a * a, dxb = b * b;
+        const double dx1 = px - L.x1, dx2 = L.x2 - px;
+        const double dxa = dx1 * dx1, dxb = dx2 * dx2;
+        // screening pass: bit jj of pass set when candidate (i, center_out(jj))
+        // may still be the minimum
+        unsigned pass = 0u;
 #pragma unroll
-        for (int j = 0; j < G; ++j) {
-            double pz;
-            if (empty) {
-                pz = 0.0;
-            } else {
-                const double* r0 = d.Z + jr0[j] + i0;
-                const double* r1 = d.Z + jr1[j] + i0;
-                double z00 = r0[0], z0b = r0[1], z10 = r1[0], z1b = r1[1];
-                double z01 = di ? z0b : z00;
-                double z11 = di ? z1b : z10;
-                if (!di && jr0[j] == jr1[j] && jr0[j] == 0) {
+        for (int jj = 0; jj < G; ++jj) {
+            const int j = center_out(jj, C);
+            if ((double)lbc[j] > Mrow) continue;
+            double ze = 0.0;
+            if (!empty) {
+                const int r0o = jr[j] >> 1;
+                const int r1o = r0o + ((jr[j] & 1) ? d.zs : 0);
+                const double* r0 = d.Z + r0o + i0;
+                const double* r1 = d.Z + r1o + i0;
+                const double z00 = r0[0], z10 = r1[0];
+                const double z01 = di ? r0[1] : z00;
+                const double z11 = di ? r1[1] : z10;
+                const double za = __builtin_fma(z01 - z00, wx, z00);
+                const double zb = __builtin_fma(z11 - z10, wx, z10);
+                ze = __builtin_fma(zb - za, wy[j], za);
+            }
+            const double dy1 = pyv[j] - L.y1, dy2 = L.y2 - pyv[j];
+            const double e1 = ze - L.z1, e2 = L.z2 - ze;
+            const double A = __builtin_fma(e1, e1, __builtin_fma(dy1, dy1, dxa));
+            const double B = __builtin_fma(e2, e2, __builtin_fma(dy2, dy2, dxb));
+            const double os = __builtin_fma(n2, sqrt_est(B), n1 * sqrt_est(A));
+            runS = fmin(runS, os);
+            const double M = fmin(fmin(minOpt, lvMin), (runS + L.D) * (1.0 + 3.0 * kRho));
+            if (!(L.screen && os > M * (1.0 + 2.0 * kRho) + L.D)) pass |= 1u << jj;
+            // keep the scheduler from hoisting every column's loads (register pressure)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // exact evaluation of the survivors, raw_data_processing.cpp:525-541
+        // operation order; the column terms are rebuilt from j exactly as above
+        const double omwx = 1 - wx;
+        while (pass) {
+            const int jj = __builtin_ctz(pass);
+            pass &= pass - 1u;
+            const int j = center_out(jj, C);
+            const double py = minY + j * spanY / (G - 1);
+            double pz = 0.0;
+            if (!empty) {
+                int j0 = 0, dj = 0;
+                double w = 0.0;
+                axis_lookup(d.Y, d.ny, d.y0, d.ihy, ysorted, py, j0, dj, w);
+                const double* r0 = d.Z + j0 * d.zs + i0;
+                const double* r1 = d.Z + (j0 + dj) * d.zs + i0;
+                const double z00 = r0[0], z10 = r1[0];
+                const double z01 = di ? r0[1] : z00;
+                const double z11 = di ? r1[1] : z10;
+                if (!di && !dj) {
                     pz = z00;
                 } else {
-                    const double omwy = 1 - wy[j];
-                    pz = z00 * omwx * omwy + z01 * wx * omwy + z10 * omwx * wy[j] + z11 * wx * wy[j];
+                    const double omwy = 1 - w;
+                    pz = z00 * omwx * omwy + z01 * wx * omwy + z10 * omwx * w + z11 * wx * w;
                 }
             }
-            const double c1 = pz - z1;
-            const double d1 = __builtin_sqrt(dxa + dya[j] + c1 * c1);
-            const double c2 = z2 - pz;
-            const double d2 = __builtin_sqrt(dxb + dyb[j] + c2 * c2);
+            const double dy1 = py - L.y1, dy2 = L.y2 - py;
+            const double c1 = pz - L.z1;
+            const double d1 = __builtin_sqrt(dxa + dy1 * dy1 + c1 * c1);
+            const double c2 = L.z2 - pz;
+            const double d2 = __builtin_sqrt(dxb + dy2 * dy2 + c2 * c2);
             const double opt = n1 * d1 + n2 * d2;
-            if (opt < minOpt) { minOpt = opt; bi = i; bj = j; }
+            const int kr = i * G + j;
+            if (opt < lvMin || (opt == lvMin && kr < lvK)) { lvMin = opt; lvK = kr; }
         }
     }
-    if (bi >= 0) {
+    if (lvMin < minOpt) {
+        minOpt = lvMin;
+        const int bi = lvK / G, bj = lvK - (lvK / G) * G;
         bestX = minX + bi * spanX / (G - 1);
         bestY = minY + bj * spanY / (G - 1);
     }
@@ -168,11 +278,12 @@ __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double 
     const double ex = x2 - x1, ey = y2 - y1;
     double R = std_max(__builtin_sqrt(ex * ex + ey * ey), 500.0);    // :483-484
     double minOpt = 1e9;                                             // :497
-    fermat_level<15>(d, x1, y1, z1, x2, y2, z2, n1, n2, R, bestX, bestY, minOpt);
+    const Leg L = make_leg(d, x1, y1, z1, x2, y2, z2, n1, n2);
+    fermat_level<15>(d, L, R, bestX, bestY, minOpt);
     R /= 5.0;                                                        // :551
 #pragma unroll 1
     for (int level = 1; level < 5; ++level) {
-        fermat_level<11>(d, x1, y1, z1, x2, y2, z2, n1, n2, R, bestX, bestY, minOpt);
+        fermat_level<11>(d, L, R, bestX, bestY, minOpt);
         R /= 3.0;
     }
     qx = bestX;                                                      // :555-557

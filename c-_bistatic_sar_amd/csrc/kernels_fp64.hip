@@ -26,7 +26,7 @@ __global__ __launch_bounds__(256) void k_dem_interp(DemDev dem, const double* __
     z[k] = dem_interp(dem, xy[2 * k], xy[2 * k + 1]);
 }
 
-__global__ __launch_bounds__(256) void k_refraction(DemDev dem, const double* __restrict__ legs, int64_t m,
+__global__ __launch_bounds__(256, 2) void k_refraction(DemDev dem, const double* __restrict__ legs, int64_t m,
                                                     double n1, double n2, double* __restrict__ out)
 {
     extern __shared__ double lds[];
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_refraction(DemDev dem, const double* __
 
 // Thread per (point v, pulse i): both legs of one calculaSlantRangeFermat
 // iteration (raw_data_processing.cpp:166-181).
-__global__ __launch_bounds__(256) void k_slant_range(DemDev dem, Traj tr, const double* __restrict__ pts,
+__global__ __launch_bounds__(256, 2) void k_slant_range(DemDev dem, Traj tr, const double* __restrict__ pts,
                                                      int64_t V, double n1, double n2,
                                                      double* __restrict__ ida, double* __restrict__ volta)
 {
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_spline(const double2* __restrict__ in, 
 // this launch's window in increasing order -- with one group that is exactly
 // the reference's sequential accumulation (proc_test_script.cpp:589-641).
 template <bool kLds>
-__global__ __launch_bounds__(256) void k_bp_fp64(BpArgs a)
+__global__ __launch_bounds__(256, 2) void k_bp_fp64(BpArgs a)
 {
     extern __shared__ double lds[];
     DemDev d = a.dem;

@@ -99,7 +99,7 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
 }
 
 template <bool kLds>
-__global__ __launch_bounds__(256) void k_bp_mixed(BpArgs a, MixedArgs m)
+__global__ __launch_bounds__(256, 2) void k_bp_mixed(BpArgs a, MixedArgs m)
 {
     extern __shared__ double lds[];
     DemDev d = a.dem;
