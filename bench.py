@@ -122,7 +122,9 @@ def cpu_baseline(sc, root, target_s: float = 20.0) -> dict | None:
     built from the reference sources) on a strided voxel subset of the same
     workload; falls back to the C restatement (oracle/liboracle.so)."""
     ref = ROOT / "oracle" / "_ref" / "proc_test_script"
-    nx, ny, nz = 4, 4, 8
+    # about 15 s of the reference's 12-thread voxel loop at 2048 pulses
+    side = 12 if sc.n >= 1024 else 16
+    nx, ny, nz = side, side, side
     sub = lambda a, m: a[np.linspace(0, len(a) - 1, m).round().astype(int)]
     gx, gy, gz = sub(sc.gx, nx), sub(sc.gy, ny), sub(sc.gz, nz)
     vox = nx * ny * nz

@@ -137,6 +137,37 @@ __device__ __forceinline__ Leg make_leg(const DemDev& d, double x1, double y1, d
 
 __device__ __forceinline__ int center_out(int k, int c) { return (k & 1) ? c - ((k + 1) >> 1) : c + (k >> 1); }
 
+// Exact optical path of candidate (px, py) in the reference's operation order
+// (raw_data_processing.cpp:525-534 with interpolateDEMElevation :615-680);
+// i0, di, wx are px's x-interval terms, dxa = (px - x1)^2, dxb = (x2 - px)^2.
+__device__ __forceinline__ double exact_opt(const DemDev& d, const Leg& L, bool empty, bool ysorted, int i0, int di,
+                                            double wx, double dxa, double dxb, double py)
+{
+    double pz = 0.0;
+    if (!empty) {
+        int j0 = 0, dj = 0;
+        double w = 0.0;
+        axis_lookup(d.Y, d.ny, d.y0, d.ihy, ysorted, py, j0, dj, w);
+        const double* r0 = d.Z + j0 * d.zs + i0;
+        const double* r1 = d.Z + (j0 + dj) * d.zs + i0;
+        const double z00 = r0[0], z10 = r1[0];
+        const double z01 = di ? r0[1] : z00;
+        const double z11 = di ? r1[1] : z10;
+        if (!di && !dj) {
+            pz = z00;
+        } else {
+            const double omwx = 1 - wx, omwy = 1 - w;
+            pz = z00 * omwx * omwy + z01 * wx * omwy + z10 * omwx * w + z11 * wx * w;
+        }
+    }
+    const double dy1 = py - L.y1, dy2 = L.y2 - py;
+    const double c1 = pz - L.z1;
+    const double d1 = __builtin_sqrt(dxa + dy1 * dy1 + c1 * c1);
+    const double c2 = L.z2 - pz;
+    const double d2 = __builtin_sqrt(dxb + dy2 * dy2 + c2 * c2);
+    return L.n1 * d1 + L.n2 * d2;
+}
+
 template <int G>
 __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, double R, double& bestX, double& bestY,
                                              double& minOpt)
@@ -149,14 +180,47 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     const bool ysorted = (d.flags & DEM_Y_SORTED) != 0;
     const bool xsorted = (d.flags & DEM_X_SORTED) != 0;
     const double n1 = L.n1, n2 = L.n2;
+    const double stepX = spanX / (G - 1), stepY = spanY / (G - 1);
+    const double epsx = 1e-12 * (fabs(minX) + fabs(spanX) + 1.0);
+    const double epsy = 1e-12 * (fabs(minY) + fabs(spanY) + 1.0);
 
-    // Per-column terms (exact) and column lower bounds.  jr[j] is the Z
-    // offset of row j0 with bit 0 set when the y interval was found (dj = 1).
+    // Probe: the centre candidate (the previous level's best) evaluated
+    // exactly first, so the bounds below start from a tight value.
+    double lvMin, M;
+    int lvK = C * G + C;
+    {
+        const double px = minX + C * spanX / (G - 1);
+        int i0 = 0, di = 0;
+        double wx = 0.0;
+        if (!empty) axis_lookup(d.X, d.nx, d.x0, d.ihx, xsorted, px, i0, di, wx);
+        const double dx1 = px - L.x1, dx2 = L.x2 - px;
+        lvMin = exact_opt(d, L, empty, ysorted, i0, di, wx, dx1 * dx1, dx2 * dx2, minY + C * spanY / (G - 1));
+        if (!(lvMin == lvMin)) {          // a NaN path never wins (strict '<' in the reference)
+            lvMin = __builtin_inf();
+            lvK = 0x7fffffff;
+        }
+        M = L.screen ? fmin(minOpt, lvMin) : __builtin_inf();
+    }
+
+    // Per-column terms for the columns whose lower bound can still reach M;
+    // jr[j] is the Z offset of row j0 with bit 0 set when the y interval was
+    // found.  Pruned columns get lbc = +inf and are never read again.
     int jr[G];
     double wy[G], pyv[G];
     float lbc[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
+        jr[j] = 0;
+        wy[j] = 0.0;
+        pyv[j] = 0.0;
+        const double pya = __builtin_fma((double)j, stepY, minY);
+        const double a = fmax(fabs(pya - L.y1) - epsy, 0.0), b = fmax(fabs(pya - L.y2) - epsy, 0.0);
+        const double lb = (n1 * sqrt_est(__builtin_fma(a, a, L.g1sq)) + n2 * sqrt_est(__builtin_fma(b, b, L.g2sq))) * kDefl;
+        if (lb > M) {
+            lbc[j] = __builtin_inff();
+            continue;
+        }
+        lbc[j] = (float)lb;   // rounding up by < 6e-8 relative stays inside the kDefl margin
         const double py = minY + j * spanY / (G - 1);
         int j0 = 0, dj = 0;
         double w = 0.0;
@@ -164,21 +228,12 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         jr[j] = ((j0 * d.zs) << 1) | dj;
         wy[j] = w;
         pyv[j] = py;
-        const double a = py - L.y1, b = L.y2 - py;
-        const double lb = n1 * sqrt_est(__builtin_fma(a, a, L.g1sq)) + n2 * sqrt_est(__builtin_fma(b, b, L.g2sq));
-        lbc[j] = (float)(lb * kDefl);
     }
-    const double stepX = spanX / (G - 1);
-    const double epsx = 1e-12 * (fabs(minX) + fabs(spanX) + 1.0);
-    double lvMin = __builtin_inf();
-    int lvK = 0x7fffffff;
-    double runS = __builtin_inf();
 
+    double runS = __builtin_inf();
 #pragma unroll 1
     for (int k = 0; k < G; ++k) {
         const int i = center_out(k, C);
-        const double Mrow = L.screen ? fmin(fmin(minOpt, lvMin), (runS + L.D) * (1.0 + 3.0 * kRho))
-                                     : __builtin_inf();
         // row lower bound from an approximate px (|error| < epsx)
         {
             const double pxa = __builtin_fma((double)i, stepX, minX);
@@ -186,7 +241,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
             const double a2 = fmax(fabs(pxa - L.x2) - epsx, 0.0);
             const double lbr = (n1 * sqrt_est(__builtin_fma(a1, a1, L.g1sq)) +
                                 n2 * sqrt_est(__builtin_fma(a2, a2, L.g2sq))) * kDefl;
-            if (lbr > Mrow) continue;
+            if (lbr > M) continue;
         }
         const double px = minX + i * spanX / (G - 1);
         int i0 = 0, di = 0;
@@ -194,13 +249,14 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         if (!empty) axis_lookup(d.X, d.nx, d.x0, d.ihx, xsorted, px, i0, di, wx);
         const double dx1 = px - L.x1, dx2 = L.x2 - px;
         const double dxa = dx1 * dx1, dxb = dx2 * dx2;
-        // screening pass: bit jj of pass set when candidate (i, center_out(jj))
-        // may still be the minimum
+        // screening pass: bit jj set when candidate (i, center_out(jj)) may
+        // still be the minimum
         unsigned pass = 0u;
 #pragma unroll
         for (int jj = 0; jj < G; ++jj) {
             const int j = center_out(jj, C);
-            if ((double)lbc[j] > Mrow) continue;
+            if ((double)lbc[j] > M) continue;
+            if (i == C && j == C) continue;          // the probe
             double ze = 0.0;
             if (!empty) {
                 const int r0o = jr[j] >> 1;
@@ -219,45 +275,22 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
             const double A = __builtin_fma(e1, e1, __builtin_fma(dy1, dy1, dxa));
             const double B = __builtin_fma(e2, e2, __builtin_fma(dy2, dy2, dxb));
             const double os = __builtin_fma(n2, sqrt_est(B), n1 * sqrt_est(A));
-            runS = fmin(runS, os);
-            const double M = fmin(fmin(minOpt, lvMin), (runS + L.D) * (1.0 + 3.0 * kRho));
-            if (!(L.screen && os > M * (1.0 + 2.0 * kRho) + L.D)) pass |= 1u << jj;
-            // keep the scheduler from hoisting every column's loads (register pressure)
-            __builtin_amdgcn_sched_barrier(0);
+            if (L.screen) {
+                runS = fmin(runS, os);
+                M = fmin(M, (runS + L.D) * (1.0 + 3.0 * kRho));
+                if (os > M * (1.0 + 2.0 * kRho) + L.D) continue;
+            }
+            pass |= 1u << jj;
         }
-        // exact evaluation of the survivors, raw_data_processing.cpp:525-541
-        // operation order; the column terms are rebuilt from j exactly as above
-        const double omwx = 1 - wx;
+        // exact evaluation of the survivors
         while (pass) {
             const int jj = __builtin_ctz(pass);
             pass &= pass - 1u;
             const int j = center_out(jj, C);
-            const double py = minY + j * spanY / (G - 1);
-            double pz = 0.0;
-            if (!empty) {
-                int j0 = 0, dj = 0;
-                double w = 0.0;
-                axis_lookup(d.Y, d.ny, d.y0, d.ihy, ysorted, py, j0, dj, w);
-                const double* r0 = d.Z + j0 * d.zs + i0;
-                const double* r1 = d.Z + (j0 + dj) * d.zs + i0;
-                const double z00 = r0[0], z10 = r1[0];
-                const double z01 = di ? r0[1] : z00;
-                const double z11 = di ? r1[1] : z10;
-                if (!di && !dj) {
-                    pz = z00;
-                } else {
-                    const double omwy = 1 - w;
-                    pz = z00 * omwx * omwy + z01 * wx * omwy + z10 * omwx * w + z11 * wx * w;
-                }
-            }
-            const double dy1 = py - L.y1, dy2 = L.y2 - py;
-            const double c1 = pz - L.z1;
-            const double d1 = __builtin_sqrt(dxa + dy1 * dy1 + c1 * c1);
-            const double c2 = L.z2 - pz;
-            const double d2 = __builtin_sqrt(dxb + dy2 * dy2 + c2 * c2);
-            const double opt = n1 * d1 + n2 * d2;
+            const double opt = exact_opt(d, L, empty, ysorted, i0, di, wx, dxa, dxb, minY + j * spanY / (G - 1));
             const int kr = i * G + j;
             if (opt < lvMin || (opt == lvMin && kr < lvK)) { lvMin = opt; lvK = kr; }
+            if (L.screen) M = fmin(M, lvMin);
         }
     }
     if (lvMin < minOpt) {
