@@ -62,6 +62,14 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # host tools over the C ABI (no device code): g++, rpath to the library
+    for tool in sorted((PKG / "tools").glob("*.cpp")):
+        exe = PKG / tool.stem
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(tool), "-o", str(exe),
+               f"-L{PKG}", "-lbsar", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed on {tool.name}:\n{r.stderr}")
     if verbose:
         print(f"built {LIB.relative_to(ROOT)} from {len(srcs)} sources", flush=True)
     return LIB

@@ -1,0 +1,102 @@
+"""The drop-in layer: bsar_proc (replaces proc_test_script on the same file
+set) and the C++ facade with the reference's function names."""
+import subprocess
+
+import numpy as np
+import pytest
+
+from _files import write_proc_dir
+from conftest import GOLDEN, PKG_DIR, ROOT
+from test_oracle_golden import parse_image
+
+TOOL = PKG_DIR / "bsar_proc"
+
+
+def run_tool(*args, cwd=None):
+    return subprocess.run([str(TOOL), *map(str, args)], capture_output=True, text=True, timeout=300, cwd=cwd)
+
+
+def test_tool_parses_reference_inputs(tmp_path):
+    g = np.load(GOLDEN / "ref_bp_random.npz")
+    d = tmp_path / "build" / "output" / "proc"
+    write_proc_dir(d, g)
+    r = run_tool("--check", cwd=tmp_path)          # default directory, as the reference
+    assert r.returncode == 0, r.stderr
+    N, S = g["root"].shape
+    assert f"rootData {S} x {N}" in r.stdout
+    assert f"grid {len(g['gx'])} x {len(g['gy'])} x {len(g['gz'])}" in r.stdout
+    assert f"{len(g['tx'])} positions" in r.stdout
+
+
+@pytest.mark.parametrize("missing", ["rootData_complex.bin", "grid_yAxis.csv", "Tx_positions.csv"])
+def test_tool_missing_input_exits_1(tmp_path, missing):
+    g = np.load(GOLDEN / "ref_bp_random.npz")
+    write_proc_dir(tmp_path, g)
+    (tmp_path / missing).unlink()
+    r = run_tool("--check", "--dir", tmp_path)
+    assert r.returncode == 1 and "[ERROR]" in r.stderr
+
+
+def test_tool_rejects_non_2d_rootdata(tmp_path):
+    g = np.load(GOLDEN / "ref_bp_random.npz")
+    write_proc_dir(tmp_path, g)
+    import struct
+    (tmp_path / "rootData_complex.bin").write_bytes(
+        struct.pack("<III", 1, 1, 6) + b"double".ljust(16, b"\0") + struct.pack("<Q", 4) + np.zeros(8).tobytes())
+    r = run_tool("--check", "--dir", tmp_path)
+    assert r.returncode == 1
+
+
+def demo_exe(tmp_path):
+    exe = tmp_path / "facade_demo"
+    r = subprocess.run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(ROOT / "tests" / "facade_demo.cpp"),
+                        "-o", str(exe), f"-L{PKG_DIR}", "-lbsar", f"-Wl,-rpath,{PKG_DIR}"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_facade_compiles_against_reference_style_types(tmp_path):
+    assert demo_exe(tmp_path).exists()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tag", ["random", "target"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tool_output_matches_reference_binary(tmp_path, tag, precision):
+    g = np.load(GOLDEN / f"ref_bp_{tag}.npz")
+    write_proc_dir(tmp_path, g)
+    r = run_tool("--dir", tmp_path, "--precision", precision)
+    assert r.returncode == 0, r.stdout + r.stderr
+    blob = (tmp_path / "output_3D_matrix.bin").read_bytes()
+    ref_blob = g["out_bin"].tobytes()
+    assert blob[:52] == ref_blob[:52]                 # identical header
+    img, ref = parse_image(blob), parse_image(ref_blob)
+    err = np.abs(img - ref).max() / np.abs(ref).max()
+    assert err <= (1e-9 if precision == "fp64" else 1e-4), err
+    for a, text in zip("xyz", g["grid_csv"]):
+        assert (tmp_path / f"output_3D_matrix_grid_{a}.csv").read_text() == str(text)
+
+
+@pytest.mark.gpu
+def test_facade_slant_range_bit_exact(tmp_path, golden):
+    g = golden
+    exe = demo_exe(tmp_path)
+    x, y, z = g["dem_bumps_x"], g["dem_bumps_y"], g["dem_bumps_z"]
+    f = lambda a: " ".join(float(v).hex() for v in np.ravel(a))
+    lines = [f"{len(x)} {len(y)}", f(x), f(y)] + [f(row) for row in z]
+    lines += [str(len(g["sr_tx"]))] + [f(r) for r in g["sr_tx"]] + [f(r) for r in g["sr_rx"]]
+    lines += [f(g["sr_P"]), f([1.0, 2.449489742783178])]
+    r = subprocess.run([str(exe)], input="\n".join(lines) + "\n", capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [list(map(float.fromhex, ln.split())) for ln in r.stdout.strip().splitlines()]
+    vals = np.array(rows[:-1])
+    assert np.array_equal(vals[:, 0:3], g["sr_ida"]) and np.array_equal(vals[:, 3:6], g["sr_volta"])
+    assert np.array_equal(vals[:, 6:10], g["sr_r4"].T)
+
+
+def test_facade_demo_rejects_malformed_input(tmp_path):
+    exe = demo_exe(tmp_path)
+    for bad in ("np.float64(1.0) 2\n", "3 3\n1 2\n", "2 2\n0 1\n0 1\n0 0\n0 0\n999999999\n"):
+        r = subprocess.run([str(exe)], input=bad, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2
