@@ -54,6 +54,30 @@ class ReportC(C.Structure):
                 ("pulses", C.c_int32), ("launches", C.c_int32), ("r1_exact_frac", C.c_double)]
 
 
+class ServerConfigC(C.Structure):
+    _fields_ = [("max_jobs", C.c_int), ("reserved0", C.c_int), ("hbm_cap_bytes", C.c_int64),
+                ("round_seconds", C.c_double), ("reserved", C.c_double * 4)]
+
+
+class JobDescC(C.Structure):
+    _fields_ = [("ds", C.c_void_p), ("pulses", C.c_int), ("precision", C.c_int),
+                ("gx", _dp), ("nx", C.c_int), ("gy", _dp), ("ny", C.c_int), ("gz", _dp), ("nz", C.c_int),
+                ("out", _dp)]
+
+
+class JobInfoC(C.Structure):
+    _fields_ = [("id", C.c_int64), ("state", C.c_int), ("status", C.c_int), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("voxel_pulses", C.c_double),
+                ("r1_exact_frac", C.c_double), ("footprint_bytes", C.c_int64)]
+
+
+class ServerStatsC(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("rejected", C.c_int64), ("rounds", C.c_int64), ("items", C.c_int64),
+                ("device_seconds", C.c_double), ("voxel_pulses", C.c_double), ("hbm_in_use", C.c_int64),
+                ("hbm_peak", C.c_int64), ("active", C.c_int), ("queued", C.c_int)]
+
+
 # name -> (restype, argtypes); every function declared in include/bsar.h
 SIGNATURES = {
     "bsar_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
@@ -83,6 +107,15 @@ SIGNATURES = {
     "bsar_write_image": (C.c_int, [C.c_char_p, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
     "bsar_write_matrix2d": (C.c_int, [C.c_char_p, _dp, C.c_uint64, C.c_uint64]),
     "bsar_flops_per_voxel_pulse": (C.c_double, [C.c_int]),
+    "bsar_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfigC), C.POINTER(C.c_void_p)]),
+    "bsar_server_destroy": (None, [C.c_void_p]),
+    "bsar_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "bsar_server_clock": (C.c_double, [C.c_void_p]),
+    "bsar_server_submit": (C.c_int, [C.c_void_p, C.POINTER(JobDescC), C.POINTER(C.c_int64)]),
+    "bsar_server_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(JobInfoC)]),
+    "bsar_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(JobInfoC)]),
+    "bsar_server_drain": (C.c_int, [C.c_void_p]),
+    "bsar_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStatsC)]),
 }
 
 _lib = None
@@ -92,9 +125,10 @@ _live = weakref.WeakSet()     # open contexts and datasets, closed before interp
 @atexit.register
 def _close_all():
     objs = list(_live)
-    for o in objs:                       # datasets first, then their contexts
-        if isinstance(o, Dataset):
-            o.close()
+    for kind in (Server, Dataset):       # servers, then datasets, then contexts
+        for o in objs:
+            if isinstance(o, kind):
+                o.close()
     for o in objs:
         o.close()
 
@@ -347,3 +381,98 @@ def write_matrix2d(path, data):
 
 def flops_per_voxel_pulse(n: int) -> float:
     return float(lib().bsar_flops_per_voxel_pulse(n))
+
+
+JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+
+
+def _info(ci: JobInfoC) -> dict:
+    d = {f: getattr(ci, f) for f, _ in JobInfoC._fields_}
+    d["state"] = JOB_STATES.get(d["state"], d["state"])
+    d["latency"] = d["t_done"] - d["t_submit"]
+    return d
+
+
+class Server:
+    """bsar_server: continuous batching of imaging jobs on one GPU.
+
+    ``submit`` queues a job (image of a grid over the first K pulses of a
+    resident dataset) and returns its id; ``wait`` returns (image [nz][ny][nx]
+    complex, info dict) once it is done.  Datasets must outlive their jobs.
+    """
+
+    def __init__(self, device: int = 0, max_jobs: int = 0, hbm_cap_bytes: int = 0, round_seconds: float = 0.0):
+        self.L = lib()
+        cfg = ServerConfigC(max_jobs, 0, int(hbm_cap_bytes), float(round_seconds))
+        h = C.c_void_p()
+        st = self.L.bsar_server_create(device, C.byref(cfg), C.byref(h))
+        if st != 0:
+            raise BsarError(st, f"bsar_server_create(device={device})")
+        self.h = h
+        self._jobs = {}          # id -> (out array, axes, dataset): kept alive until collected
+        _live.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bsar_server_destroy(self.h)
+            self.h = None
+            self._jobs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _err(self) -> str:
+        return self.L.bsar_server_last_error(self.h).decode()
+
+    def clock(self) -> float:
+        return float(self.L.bsar_server_clock(self.h))
+
+    def submit(self, ds: Dataset, gx, gy, gz, pulses: int = 0, precision: int = FP64) -> int:
+        gx, gy, gz = _f64(gx), _f64(gy), _f64(gz)
+        out = np.empty((len(gz), len(gy), len(gx)), np.complex128)
+        d = JobDescC(ds.h, int(pulses), int(precision), _ptr(gx), len(gx), _ptr(gy), len(gy), _ptr(gz), len(gz),
+                     _ptr(out.view(np.float64)))
+        jid = C.c_int64()
+        st = self.L.bsar_server_submit(self.h, C.byref(d), C.byref(jid))
+        if st != 0:
+            raise BsarError(st, f"server_submit: {self._err()}")
+        self._jobs[jid.value] = (out, ds)
+        return jid.value
+
+    def poll(self, jid: int) -> dict:
+        ci = JobInfoC()
+        st = self.L.bsar_server_poll(self.h, jid, C.byref(ci))
+        if st != 0:
+            raise BsarError(st, f"server_poll: {self._err()}")
+        return _info(ci)
+
+    def wait(self, jid: int, timeout: float = -1.0):
+        ci = JobInfoC()
+        st = self.L.bsar_server_wait(self.h, jid, float(timeout), C.byref(ci))
+        if st == -7 and ci.state not in (2, 3):
+            raise TimeoutError(f"job {jid} not done after {timeout} s")
+        out, _ = self._jobs.pop(jid, (None, None))
+        if st != 0:
+            raise BsarError(st, f"job {jid}: {self._err()}")
+        return out, _info(ci)
+
+    def drain(self):
+        st = self.L.bsar_server_drain(self.h)
+        if st != 0:
+            raise BsarError(st, f"server_drain: {self._err()}")
+
+    def stats(self) -> dict:
+        cs = ServerStatsC()
+        st = self.L.bsar_server_get_stats(self.h, C.byref(cs))
+        if st != 0:
+            raise BsarError(st, "server_stats")
+        return {f: getattr(cs, f) for f, _ in ServerStatsC._fields_}

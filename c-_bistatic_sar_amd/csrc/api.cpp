@@ -149,55 +149,69 @@ static int check_axes(bsar_ctx* ctx, const double* gx, int nx, const double* gy,
     return BSAR_OK;
 }
 
-static int backproject_fp64(bsar_ctx* ctx, const bsar_dataset* ds, const double* gx, int nx, const double* gy,
-                            int ny, const double* gz, int nz, double2* d_out, bsar_report* rep)
+BpArgs make_bp_args(const bsar_dataset* ds, int K, const double* dg, int nx, int ny, int nz, const BpPlan& pl)
 {
-    const int64_t V = (int64_t)nx * ny * nz;
-    const int n = ds->n;
-    const double cost = 1418.0 * 60.0 + 2.0 * n * 26.0 + 200.0;
-    BpPlan pl = plan_bp(V, n, cost);
-    DevBuf axes, accb;
-    BSAR_HIP(ctx, hipMalloc(&axes.p, (size_t)(nx + ny + nz) * sizeof(double)));
-    double* dgx = (double*)axes.p;
-    BSAR_HIP(ctx, hipMemcpyAsync(dgx, gx, nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    BSAR_HIP(ctx, hipMemcpyAsync(dgx + nx, gy, ny * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    BSAR_HIP(ctx, hipMemcpyAsync(dgx + nx + ny, gz, nz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    double2* acc = d_out;
-    int64_t stride = V;
-    if (pl.groups > 1) {
-        BSAR_HIP(ctx, hipMalloc(&accb.p, (size_t)pl.groups * V * sizeof(double2)));
-        acc = (double2*)accb.p;
-    }
-    BSAR_HIP(ctx, fill_zero(acc, (int64_t)pl.groups * stride, ctx->stream));
-
     BpArgs a{};
     a.dem = ds->dem;
     a.traj = ds->traj;
+    a.traj.n = K;                                  // first K pulses; R1 means over those
     a.rc = ds->rc;
-    a.rc32 = nullptr;
+    a.rc32 = ds->rc32;
     a.ndata = ds->ndata;
     a.S4 = ds->S4;
     a.s4m1 = (double)(ds->S4 - 1);
     a.n1 = ds->p.n1;
     a.n2 = ds->p.n2;
-    a.inv_c = 1.0 / ds->p.c;
-    a.fs4 = ds->p.fs * ds->p.upsample;
-    const double lambda = ds->p.c / ds->p.fc;
-    a.kphi = 2.0 * M_PI / lambda;
-    a.gx = dgx;
-    a.gy = dgx + nx;
-    a.gz = dgx + nx + ny;
+    a.inv_c = 1.0 / ds->p.c;                       // proc_test_script.cpp:567
+    a.fs4 = ds->p.fs * ds->p.upsample;             // proc_test_script.cpp:223
+    const double lambda = ds->p.c / ds->p.fc;      // proc_test_script.cpp:213
+    a.kphi = 2.0 * M_PI / lambda;                  // proc_test_script.cpp:575
+    a.gx = dg;
+    a.gy = dg + nx;
+    a.gz = dg + nx + ny;
     a.nx = nx;
     a.ny = ny;
     a.nz = nz;
     a.vox0 = 0;
-    a.nvox = V;
+    a.nvox = pl.nvox;
     a.nvox_pad = pl.nvox_pad;
     a.groups = pl.groups;
     a.glen = pl.glen;
+    a.woff = 0;
     a.wlen = pl.wlen;
+    a.acc = nullptr;
+    a.acc_stride = pl.nvox;
+    return a;
+}
+
+hipError_t finish_fp64(const double2* acc, int groups, int64_t nvox, double2* out, hipStream_t s)
+{
+    if (nvox <= 0) return hipSuccess;
+    if (groups > 1) return launch_group_sum(acc, groups, nvox, nvox, out, s);
+    if (acc == out) return hipSuccess;
+    return hipMemcpyAsync(out, acc, (size_t)nvox * sizeof(double2), hipMemcpyDeviceToDevice, s);
+}
+
+static int backproject_fp64(bsar_ctx* ctx, const bsar_dataset* ds, const double* gx, int nx, const double* gy,
+                            int ny, const double* gz, int nz, double2* d_out, bsar_report* rep)
+{
+    const int64_t V = (int64_t)nx * ny * nz;
+    const int n = ds->n;
+    BpPlan pl = plan_bp(V, n, bp_cost_fp64(n));
+    DevBuf axes, accb;
+    BSAR_HIP(ctx, hipMalloc(&axes.p, (size_t)(nx + ny + nz + 1) * sizeof(double)));
+    double* dgx = (double*)axes.p;
+    BSAR_HIP(ctx, hipMemcpyAsync(dgx, gx, nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    BSAR_HIP(ctx, hipMemcpyAsync(dgx + nx, gy, ny * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    BSAR_HIP(ctx, hipMemcpyAsync(dgx + nx + ny, gz, nz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    double2* acc = d_out;
+    if (pl.groups > 1) {
+        BSAR_HIP(ctx, hipMalloc(&accb.p, (size_t)pl.groups * V * sizeof(double2)));
+        acc = (double2*)accb.p;
+    }
+    BSAR_HIP(ctx, fill_zero(acc, (int64_t)pl.groups * V, ctx->stream));
+    BpArgs a = make_bp_args(ds, n, dgx, nx, ny, nz, pl);
     a.acc = acc;
-    a.acc_stride = stride;
 
     int launches = 0;
     BSAR_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -208,7 +222,7 @@ static int backproject_fp64(bsar_ctx* ctx, const bsar_dataset* ds, const double*
             ++launches;
         }
         if (pl.groups > 1) {
-            BSAR_HIP(ctx, launch_group_sum(acc, pl.groups, stride, V, d_out, ctx->stream));
+            BSAR_HIP(ctx, finish_fp64(acc, pl.groups, V, d_out, ctx->stream));
             ++launches;
         }
     }

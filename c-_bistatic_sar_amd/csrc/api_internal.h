@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,8 +37,9 @@ struct bsar_dataset {
     float2* rc32 = nullptr;
     int ndata = 0, S4 = 0;
     int64_t bytes = 0;
-    // R1-mean tables (mixed form), built lazily per dataset
+    // R1-mean tables (mixed form), built lazily per (dataset, pulse count)
     void* r1tab = nullptr;
+    std::mutex tab_mu;               // guards r1tab (the serving thread shares datasets)
 };
 
 extern "C" void bsar_r1tab_free(bsar_dataset* ds);
@@ -58,15 +60,39 @@ int hip_fail(bsar_ctx* ctx, hipError_t e, const char* what);
 int upload_dem(bsar_ctx* ctx, const bsar_dem* dem, double** dX, double** dY, double** dZ, DemDev* out,
                double* zmin, double* zmax);
 
-// Mixed-precision back-projection (kernels_mixed.hip / mixed.cpp).
-int backproject_mixed(bsar_ctx* ctx, const bsar_dataset* ds, const double* gx, int nx, const double* gy, int ny,
-                      const double* gz, int nz, double2* d_out, bsar_report* rep);
-
-// Work-splitting of one back-projection over launches.
+// Work-splitting of one back-projection: pulse groups (partial images summed
+// in group order at the end) and the pulse window of one launch.  The group
+// split depends only on (nvox, n_pulses), so every caller that plans the
+// same image sums each voxel's pulses in the same order.
 struct BpPlan {
     int groups = 1, glen = 0, wlen = 0;
     int64_t nvox = 0, nvox_pad = 0;
 };
 BpPlan plan_bp(int64_t nvox, int n_pulses, double cost_per_vp);
+
+// Lane-instruction estimates per voxel-pulse (launch sizing and scheduling).
+inline double bp_cost_fp64(int n) { return 1418.0 * 60.0 + 2.0 * n * 26.0 + 200.0; }
+inline double bp_cost_mixed() { return 1418.0 * 60.0 + 600.0; }
+
+// Argument block of one image over the first K pulses of ds (device axes
+// dg = gx | gy | gz) with plan pl; accumulators are set by the caller.
+BpArgs make_bp_args(const bsar_dataset* ds, int K, const double* dg, int nx, int ny, int nz, const BpPlan& pl);
+
+hipError_t fill_zero(double2* p, int64_t n, hipStream_t s);
+
+// Final image from the group accumulators: out[v] = ordered sum over groups.
+// out may be acc itself (16 bytes per voxel in both forms).
+hipError_t finish_fp64(const double2* acc, int groups, int64_t nvox, double2* out, hipStream_t s);
+hipError_t finish_mixed(const float4* acc, int groups, int64_t nvox, double2* out, hipStream_t s);
+
+// Mixed-precision back-projection (kernels_mixed.hip / mixed.cpp).
+int backproject_mixed(bsar_ctx* ctx, const bsar_dataset* ds, const double* gx, int nx, const double* gy, int ny,
+                      const double* gz, int nz, double2* d_out, bsar_report* rep);
+
+// R1 tables for the first K pulses of ds, with the lookup box of this grid's
+// refraction points (built on ctx's stream when missing; ds->tab_mu held).
+// The device arrays stay valid until the dataset is destroyed.
+int r1_tables(bsar_ctx* ctx, bsar_dataset* ds, int K, const double* gx, int nx, const double* gy, int ny,
+              const double* gz, int nz, R1Table* tab, const double** ftx, const double** frx);
 
 }  // namespace bsar

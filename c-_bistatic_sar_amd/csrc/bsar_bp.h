@@ -1,0 +1,194 @@
+// bsar_bp.h -- per-voxel back-projection bodies shared by the one-image
+// kernels (kernels_fp64.hip, kernels_mixed.hip) and the persistent serving
+// kernel (kernels_serve.hip).  One call adds pulses [p0, p1) of pulse group g
+// to voxel v's running sum, in increasing pulse order.
+#pragma once
+#include <math.h>
+
+#include "bsar_device.h"
+#include "bsar_kernels.h"
+
+namespace bsar {
+
+// Plain FP64 body, proc_test_script.cpp:484-668 in the reference's operation
+// order (built with -ffp-contract=off).
+__device__ __forceinline__ void bp_fp64_thread(const DemDev& d, const BpArgs& a, int g, int64_t v, int p0, int p1)
+{
+    const int xi = (int)(v % a.nx);
+    const int yi = (int)((v / a.nx) % a.ny);
+    const int zi = (int)(v / ((int64_t)a.nx * a.ny));
+    const double Px = a.gx[xi], Py = a.gy[yi], Pz = a.gz[zi];
+    const double n1 = a.n1, n2 = a.n2;
+    const int n = a.traj.n;
+    const double dn = (double)n;
+    double2* accp = a.acc + (int64_t)g * a.acc_stride + v;
+    double accr = accp->x, acci = accp->y;
+
+    for (int i = p0; i < p1; ++i) {
+        double qax, qay, qaz, qbx, qby, qbz;
+        fermat_solve(d, a.traj.tx[i], a.traj.ty[i], a.traj.tz[i], Px, Py, Pz, n1, n2, qax, qay, qaz);   // rdp:166-172
+        fermat_solve(d, Px, Py, Pz, a.traj.rx[i], a.traj.ry[i], a.traj.rz[i], n2, n1, qbx, qby, qbz);   // rdp:175-181
+        double e0 = Px - qax, e1 = Py - qay, e2 = Pz - qaz;
+        const double R2t = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:496-505
+        e0 = Px - qbx; e1 = Py - qby; e2 = Pz - qbz;
+        const double R2r = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:507-516
+        double st = 0.0, sr = 0.0;
+        for (int j = 0; j < n; ++j) {                                                                     // proc:518-532
+            const double f0 = a.traj.tx[j] - qax, f1 = a.traj.ty[j] - qay, f2 = a.traj.tz[j] - qaz;
+            st += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+        }
+        for (int j = 0; j < n; ++j) {                                                                     // proc:534-548
+            const double f0 = a.traj.rx[j] - qbx, f1 = a.traj.ry[j] - qby, f2 = a.traj.rz[j] - qbz;
+            sr += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+        }
+        const double R1t = st / dn, R1r = sr / dn;
+        const double path = n1 * R1t + n2 * R2t + n2 * R2r + n1 * R1r;
+        const double tv = a.inv_c * path;                                                                // proc:567
+        const double rngBin = 1.0 + (tv * a.fs4);                                                        // proc:571
+        const double phi = a.kphi * path;                                                                // proc:575
+        const double r = rngBin - 1.0;                                                                   // proc:593
+        if (!(r >= 0.0 && r <= a.s4m1)) continue;                                                        // proc:595-599
+        const int iL = (int)floor(r);
+        const int iN = (int)ceil(r);
+        const double q = r - (double)iL;                                                                 // proc:602
+        const int az = min(i, a.ndata - 1);                                                              // proc:607
+        const double2* row = a.rc + (int64_t)az * a.S4;
+        const double2 L = row[iL];
+        double vr, vi;
+        if (iL == iN) {                                                                                  // proc:626-628
+            vr = L.x; vi = L.y;
+        } else {                                                                                         // proc:632-634
+            const double2 N = row[iN];
+            vr = (1.0 - q) * L.x + q * N.x;
+            vi = (1.0 - q) * L.y + q * N.y;
+        }
+        double sn, cs;
+        sincos(phi, &sn, &cs);                                                                           // proc:638
+        accr += vr * cs - vi * sn;                                                                       // proc:641
+        acci += vr * sn + vi * cs;
+    }
+    *accp = make_double2(accr, acci);
+}
+
+__device__ __forceinline__ void lagrange4(double t, double w[4])
+{
+    const double tm1 = t - 1.0, tm2 = t - 2.0, tp1 = t + 1.0;
+    w[0] = -t * tm1 * tm2 * (1.0 / 6.0);
+    w[1] = tp1 * tm1 * tm2 * 0.5;
+    w[2] = -tp1 * t * tm2 * 0.5;
+    w[3] = tp1 * t * tm1 * (1.0 / 6.0);
+}
+
+// Tricubic Lagrange lookup; false outside the table's lookup box.  The box
+// keeps the 4x4x4 stencil inside the table ([lo, hi) within [a + 1, a + n - 2)).
+__device__ __forceinline__ bool r1_lookup(const R1Table& tb, const double* __restrict__ F, double qx, double qy,
+                                          double qz, double& val)
+{
+    const double ux = qx * tb.ih, uy = qy * tb.ih, uz = qz * tb.ih;   // exact: ih is a power of two
+    if (!(ux >= tb.lo[0] && ux < tb.hi[0] && uy >= tb.lo[1] && uy < tb.hi[1] && uz >= tb.lo[2] && uz < tb.hi[2]))
+        return false;
+    const double fx = floor(ux), fy = floor(uy), fz = floor(uz);
+    const int ix = (int)fx - 1 - tb.ax, iy = (int)fy - 1 - tb.ay, iz = (int)fz - 1 - tb.az;
+    double wx[4], wy[4], wz[4];
+    lagrange4(ux - fx, wx);
+    lagrange4(uy - fy, wy);
+    lagrange4(uz - fz, wz);
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double accy = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const double* row = F + ((int64_t)(iz + c) * tb.ny + (iy + b)) * tb.nx + ix;
+            const double lx = wx[0] * row[0] + wx[1] * row[1] + wx[2] * row[2] + wx[3] * row[3];
+            accy += wy[b] * lx;
+        }
+        acc += wz[c] * accy;
+    }
+    val = acc;
+    return true;
+}
+
+__device__ __forceinline__ double r1_exact(const double* __restrict__ ax, const double* __restrict__ ay,
+                                           const double* __restrict__ az, int n, double qx, double qy, double qz)
+{
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double f0 = ax[j] - qx, f1 = ay[j] - qy, f2 = az[j] - qz;
+        s += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+    }
+    return s / (double)n;
+}
+
+// Kahan step: (s, c) += x
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x)
+{
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Mixed-precision body: exact FP64 refraction points, FP64 delay and phase
+// argument, tabulated R1 means, FP32 interpolation and Kahan accumulation.
+__device__ __forceinline__ void bp_mixed_thread(const DemDev& d, const BpArgs& a, const MixedArgs& m, int g,
+                                                int64_t v, int p0, int p1, unsigned& exact)
+{
+    const int xi = (int)(v % a.nx);
+    const int yi = (int)((v / a.nx) % a.ny);
+    const int zi = (int)(v / ((int64_t)a.nx * a.ny));
+    const double Px = a.gx[xi], Py = a.gy[yi], Pz = a.gz[zi];
+    const double n1 = a.n1, n2 = a.n2;
+    const int n = a.traj.n;
+    float4* st = m.acc + (int64_t)g * a.acc_stride + v;
+    float4 s = *st;   // (sum.re, sum.im, comp.re, comp.im)
+
+    for (int i = p0; i < p1; ++i) {
+        double qax, qay, qaz, qbx, qby, qbz;
+        fermat_solve(d, a.traj.tx[i], a.traj.ty[i], a.traj.tz[i], Px, Py, Pz, n1, n2, qax, qay, qaz);
+        fermat_solve(d, Px, Py, Pz, a.traj.rx[i], a.traj.ry[i], a.traj.rz[i], n2, n1, qbx, qby, qbz);
+        {
+#pragma clang fp contract(fast)
+            double e0 = Px - qax, e1 = Py - qay, e2 = Pz - qaz;
+            const double R2t = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            e0 = Px - qbx; e1 = Py - qby; e2 = Pz - qbz;
+            const double R2r = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            double R1t, R1r;
+            if (!r1_lookup(m.tab, m.ftx, qax, qay, qaz, R1t)) {
+                R1t = r1_exact(a.traj.tx, a.traj.ty, a.traj.tz, n, qax, qay, qaz);
+                ++exact;
+            }
+            if (!r1_lookup(m.tab, m.frx, qbx, qby, qbz, R1r)) {
+                R1r = r1_exact(a.traj.rx, a.traj.ry, a.traj.rz, n, qbx, qby, qbz);
+                ++exact;
+            }
+            const double path = n1 * R1t + n2 * R2t + n2 * R2r + n1 * R1r;
+            const double r = (1.0 + (a.inv_c * path) * a.fs4) - 1.0;
+            if (!(r >= 0.0 && r <= a.s4m1)) continue;
+            const int iL = (int)floor(r);
+            const int iN = (int)ceil(r);
+            const float q = (float)(r - (double)iL);
+            // phase: FP64 argument reduced to [-pi, pi] in FP64, then FP32
+            const double phi = a.kphi * path;
+            const double red = phi - m.two_pi * rint(phi * m.inv_two_pi);
+            float sn, cs;
+            sincosf((float)red, &sn, &cs);
+            const int az = min(i, a.ndata - 1);
+            const float2* row = a.rc32 + (int64_t)az * a.S4;
+            const float2 L = row[iL];
+            float vr, vi;
+            if (iL == iN) {
+                vr = L.x; vi = L.y;
+            } else {
+                const float2 N = row[iN];
+                vr = (1.0f - q) * L.x + q * N.x;
+                vi = (1.0f - q) * L.y + q * N.y;
+            }
+            kahan_add(s.x, s.z, vr * cs - vi * sn);
+            kahan_add(s.y, s.w, vr * sn + vi * cs);
+        }
+    }
+    *st = s;
+}
+
+}  // namespace bsar

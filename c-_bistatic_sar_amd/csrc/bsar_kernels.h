@@ -41,6 +41,29 @@ struct BpArgs {
     int64_t acc_stride;
 };
 
+// Lattice of the mean-distance tables.  The spacing h is a power of two and
+// node (i, j, k) sits at ((ax + i) h, (ay + j) h, (az + k) h), i < nx, j < ny,
+// k < nz; values [k][j][i].  A node's value depends only on its absolute
+// lattice index, and a lookup at q uses u = q / h (exact), so any two tables
+// of one spacing give bit-identical answers wherever both cover q.  Lookups
+// answer only for u in [lo, hi) on every axis (the box of the image that
+// uses the table); everywhere else the caller sums exactly.
+struct R1Table {
+    double h, ih;
+    int ax, ay, az;
+    int nx, ny, nz;
+    double lo[3], hi[3];
+};
+
+struct MixedArgs {
+    R1Table tab;
+    const double* ftx;               // Tx mean-distance table
+    const double* frx;               // Rx mean-distance table
+    float4* acc;                     // [groups][acc_stride] (sum.re, sum.im, comp.re, comp.im)
+    unsigned long long* exact_count; // R1 evaluations that fell outside the table
+    double two_pi, inv_two_pi;
+};
+
 hipError_t launch_dem_interp(const DemDev& dem, const double* xy, int64_t m, double* z, hipStream_t s);
 hipError_t launch_refraction(const DemDev& dem, const double* legs, int64_t m, double n1, double n2,
                              double* out, hipStream_t s);

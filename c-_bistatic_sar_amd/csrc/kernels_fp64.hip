@@ -11,6 +11,7 @@
 // operation order exactly, so refraction points, distances, delays and the
 // interpolation weights are bit-identical to the reference's FP64 CPU build;
 // only sin/cos (device libm vs glibc) may differ in the last bit.
+#include "bsar_bp.h"
 #include "bsar_device.h"
 #include "bsar_kernels.h"
 
@@ -122,66 +123,13 @@ __global__ __launch_bounds__(256, 2) void k_bp_fp64(BpArgs a)
     const int p0 = gbeg + a.woff;
     const int p1 = min(p0 + a.wlen, gend);
     if (p0 >= p1) return;
-    const int64_t v = a.vox0 + vl;
-    const int xi = (int)(v % a.nx);
-    const int yi = (int)((v / a.nx) % a.ny);
-    const int zi = (int)(v / ((int64_t)a.nx * a.ny));
-    const double Px = a.gx[xi], Py = a.gy[yi], Pz = a.gz[zi];
-    const double n1 = a.n1, n2 = a.n2;
-    const int n = a.traj.n;
-    const double dn = (double)n;
-    double2* accp = a.acc + (int64_t)g * a.acc_stride + v;
-    double accr = accp->x, acci = accp->y;
-
-    for (int i = p0; i < p1; ++i) {
-        double qax, qay, qaz, qbx, qby, qbz;
-        fermat_solve(d, a.traj.tx[i], a.traj.ty[i], a.traj.tz[i], Px, Py, Pz, n1, n2, qax, qay, qaz);   // rdp:166-172
-        fermat_solve(d, Px, Py, Pz, a.traj.rx[i], a.traj.ry[i], a.traj.rz[i], n2, n1, qbx, qby, qbz);   // rdp:175-181
-        double e0 = Px - qax, e1 = Py - qay, e2 = Pz - qaz;
-        const double R2t = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:496-505
-        e0 = Px - qbx; e1 = Py - qby; e2 = Pz - qbz;
-        const double R2r = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:507-516
-        double st = 0.0, sr = 0.0;
-        for (int j = 0; j < n; ++j) {                                                                     // proc:518-532
-            const double f0 = a.traj.tx[j] - qax, f1 = a.traj.ty[j] - qay, f2 = a.traj.tz[j] - qaz;
-            st += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
-        }
-        for (int j = 0; j < n; ++j) {                                                                     // proc:534-548
-            const double f0 = a.traj.rx[j] - qbx, f1 = a.traj.ry[j] - qby, f2 = a.traj.rz[j] - qbz;
-            sr += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
-        }
-        const double R1t = st / dn, R1r = sr / dn;
-        const double path = n1 * R1t + n2 * R2t + n2 * R2r + n1 * R1r;
-        const double tv = a.inv_c * path;                                                                // proc:567
-        const double rngBin = 1.0 + (tv * a.fs4);                                                        // proc:571
-        const double phi = a.kphi * path;                                                                // proc:575
-        const double r = rngBin - 1.0;                                                                   // proc:593
-        if (!(r >= 0.0 && r <= a.s4m1)) continue;                                                        // proc:595-599
-        const int iL = (int)floor(r);
-        const int iN = (int)ceil(r);
-        const double q = r - (double)iL;                                                                 // proc:602
-        const int az = min(i, a.ndata - 1);                                                              // proc:607
-        const double2* row = a.rc + (int64_t)az * a.S4;
-        const double2 L = row[iL];
-        double vr, vi;
-        if (iL == iN) {                                                                                  // proc:626-628
-            vr = L.x; vi = L.y;
-        } else {                                                                                         // proc:632-634
-            const double2 N = row[iN];
-            vr = (1.0 - q) * L.x + q * N.x;
-            vi = (1.0 - q) * L.y + q * N.y;
-        }
-        double sn, cs;
-        sincos(phi, &sn, &cs);                                                                           // proc:638
-        accr += vr * cs - vi * sn;                                                                       // proc:641
-        acci += vr * sn + vi * cs;
-    }
-    *accp = make_double2(accr, acci);
+    bp_fp64_thread(d, a, g, a.vox0 + vl, p0, p1);
 }
 
 // out[v] = sum over groups in order; out is [nvox] complex (x-fastest voxels).
-__global__ __launch_bounds__(256) void k_group_sum(const double2* __restrict__ acc, int groups, int64_t stride,
-                                                   int64_t nvox, double2* __restrict__ out)
+// out may be acc itself (group 0): each thread reads its voxel's groups before writing.
+__global__ __launch_bounds__(256) void k_group_sum(const double2* acc, int groups, int64_t stride, int64_t nvox,
+                                                   double2* out)
 {
     int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (v >= nvox) return;

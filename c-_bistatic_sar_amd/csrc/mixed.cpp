@@ -1,6 +1,13 @@
 // mixed.cpp -- host side of the mixed-precision back-projection: the
 // per-dataset mean-distance tables (R1t / R1r of proc_test_script.cpp:518-548
 // as functions of the refraction point) and the launch sequence.
+//
+// Tables live on an absolute lattice of power-of-two spacing h (node index
+// times h is exact), keyed by (pulse count K, h).  Each image gets a lookup
+// box derived from its own grid only, so which tables happen to exist (other
+// images, other jobs of a serving run) never changes an image's bits: inside
+// its box every table of that (K, h) answers identically, outside it the
+// kernel sums exactly.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -11,20 +18,30 @@
 
 namespace bsar {
 
-struct R1Cache {
-    R1Table tab{};
+namespace {
+
+struct R1Entry {
+    int K = 0;
+    R1Table geo{};                    // h, ih, ax.., nx.. (no lookup box)
     double* ftx = nullptr;
     double* frx = nullptr;
-    bool usable = false;
-    // region the table was built for (refraction-point box)
-    double xa = 0, xb = -1, ya = 0, yb = -1, za = 0, zb = -1;
+    std::vector<double> Ft, Fr;       // host copies (self-checks)
 };
 
-static void free_cache(R1Cache* c)
+struct R1Cache {
+    std::vector<R1Entry*> entries;    // immutable once built, newest last
+    bool have_dem = false;
+    std::vector<double> X, Y, Z;      // host DEM for height ranges
+};
+
+void free_cache(R1Cache* c)
 {
     if (!c) return;
-    if (c->ftx) (void)hipFree(c->ftx);
-    if (c->frx) (void)hipFree(c->frx);
+    for (R1Entry* e : c->entries) {
+        if (e->ftx) (void)hipFree(e->ftx);
+        if (e->frx) (void)hipFree(e->frx);
+        delete e;
+    }
     delete c;
 }
 
@@ -32,9 +49,8 @@ static void free_cache(R1Cache* c)
 // bilinear result is a convex combination of the nodes its interval search
 // selects (raw_data_processing.cpp:625-677), including node 0 of an axis when
 // a coordinate falls outside it.
-static void dem_range_over_box(const bsar_dataset* ds, const std::vector<double>& X, const std::vector<double>& Y,
-                               const std::vector<double>& Z, double xa, double xb, double ya, double yb, double* lo,
-                               double* hi)
+void dem_range_over_box(const bsar_dataset* ds, const R1Cache& c, double xa, double xb, double ya, double yb,
+                        double* lo, double* hi)
 {
     if (ds->dem.flags & DEM_EMPTY) { *lo = *hi = 0.0; return; }
     const int nx = ds->dem.nx, ny = ds->dem.ny, zs = ds->dem.zs;
@@ -51,10 +67,10 @@ static void dem_range_over_box(const bsar_dataset* ds, const std::vector<double>
     };
     int i0, i1, j0, j1;
     bool wx0, wy0;
-    idx_range(X, nx, (ds->dem.flags & DEM_X_SORTED) != 0, xa, xb, &i0, &i1, &wx0);
-    idx_range(Y, ny, (ds->dem.flags & DEM_Y_SORTED) != 0, ya, yb, &j0, &j1, &wy0);
+    idx_range(c.X, nx, (ds->dem.flags & DEM_X_SORTED) != 0, xa, xb, &i0, &i1, &wx0);
+    idx_range(c.Y, ny, (ds->dem.flags & DEM_Y_SORTED) != 0, ya, yb, &j0, &j1, &wy0);
     double l = INFINITY, h = -INFINITY;
-    auto take = [&](int j, int i) { double v = Z[(size_t)j * zs + i]; l = std::min(l, v); h = std::max(h, v); };
+    auto take = [&](int j, int i) { double v = c.Z[(size_t)j * zs + i]; l = std::min(l, v); h = std::max(h, v); };
     for (int j = j0; j <= j1; ++j) {
         for (int i = i0; i <= i1; ++i) take(j, i);
         if (wx0) take(j, 0);
@@ -67,20 +83,21 @@ static void dem_range_over_box(const bsar_dataset* ds, const std::vector<double>
     *hi = h;
 }
 
-static double host_r1(const std::vector<double>& a, int n, double x, double y, double z)
+double host_r1(const std::vector<double>& a, int K, double x, double y, double z)
 {
     double s = 0.0;
-    for (int j = 0; j < n; ++j) {
+    for (int j = 0; j < K; ++j) {
         const double f0 = a[3 * j] - x, f1 = a[3 * j + 1] - y, f2 = a[3 * j + 2] - z;
         s += std::sqrt(f0 * f0 + f1 * f1 + f2 * f2);
     }
-    return s / n;
+    return s / K;
 }
 
-static bool host_lookup(const R1Table& tb, const std::vector<double>& F, double qx, double qy, double qz, double* v)
+// Host mirror of r1_lookup (bsar_bp.h) for the self-check.
+bool host_lookup(const R1Table& tb, const std::vector<double>& F, double qx, double qy, double qz, double* v)
 {
-    const double ux = (qx - tb.x0) * tb.ih, uy = (qy - tb.y0) * tb.ih, uz = (qz - tb.z0) * tb.ih;
-    if (!(ux >= 1.0 && ux < tb.nx - 2.0 && uy >= 1.0 && uy < tb.ny - 2.0 && uz >= 1.0 && uz < tb.nz - 2.0))
+    const double ux = qx * tb.ih, uy = qy * tb.ih, uz = qz * tb.ih;
+    if (!(ux >= tb.lo[0] && ux < tb.hi[0] && uy >= tb.lo[1] && uy < tb.hi[1] && uz >= tb.lo[2] && uz < tb.hi[2]))
         return false;
     auto lag = [](double t, double w[4]) {
         w[0] = -t * (t - 1) * (t - 2) / 6.0;
@@ -89,7 +106,7 @@ static bool host_lookup(const R1Table& tb, const std::vector<double>& F, double 
         w[3] = (t + 1) * t * (t - 1) / 6.0;
     };
     const double fx = std::floor(ux), fy = std::floor(uy), fz = std::floor(uz);
-    const int ix = (int)fx - 1, iy = (int)fy - 1, iz = (int)fz - 1;
+    const int ix = (int)fx - 1 - tb.ax, iy = (int)fy - 1 - tb.ay, iz = (int)fz - 1 - tb.az;
     double wx[4], wy[4], wz[4];
     lag(ux - fx, wx);
     lag(uy - fy, wy);
@@ -103,13 +120,63 @@ static bool host_lookup(const R1Table& tb, const std::vector<double>& F, double 
     return true;
 }
 
-// Makes sure ds has tables covering the refraction points of this grid.
-static int ensure_tables(bsar_ctx* ctx, bsar_dataset* ds, const double* gx, int nx, const double* gy, int ny,
-                         const double* gz, int nz, R1Cache** out)
+bool covers(const R1Table& t, int ax, int ay, int az, int nx, int ny, int nz)
+{
+    return t.ax <= ax && t.ay <= ay && t.az <= az && t.ax + t.nx >= ax + nx && t.ay + t.ny >= ay + ny &&
+           t.az + t.nz >= az + nz;
+}
+
+constexpr int64_t kMaxNodes = int64_t(1) << 24;
+
+int build_entry(bsar_ctx* ctx, const bsar_dataset* ds, int K, const R1Table& geo, R1Entry** out)
 {
     *out = nullptr;
-    const int n = ds->n;
-    // Refraction points of a 3x3x3 sample of the grid over all pulses.
+    const int64_t nodes = (int64_t)geo.nx * geo.ny * geo.nz;
+    R1Entry* e = new R1Entry();
+    e->K = K;
+    e->geo = geo;
+    struct Guard { R1Entry*& e; ~Guard() { if (e) { if (e->ftx) (void)hipFree(e->ftx); if (e->frx) (void)hipFree(e->frx); delete e; } } } g{e};
+    BSAR_CHECK_HIP(ctx, hipMalloc((void**)&e->ftx, (size_t)nodes * 8));
+    BSAR_CHECK_HIP(ctx, hipMalloc((void**)&e->frx, (size_t)nodes * 8));
+    BSAR_CHECK_HIP(ctx, launch_r1_table(geo, ds->traj.tx, ds->traj.ty, ds->traj.tz, K, e->ftx, ctx->stream));
+    BSAR_CHECK_HIP(ctx, launch_r1_table(geo, ds->traj.rx, ds->traj.ry, ds->traj.rz, K, e->frx, ctx->stream));
+    e->Ft.resize((size_t)nodes);
+    e->Fr.resize((size_t)nodes);
+    BSAR_CHECK_HIP(ctx, hipMemcpyAsync(e->Ft.data(), e->ftx, (size_t)nodes * 8, hipMemcpyDeviceToHost, ctx->stream));
+    BSAR_CHECK_HIP(ctx, hipMemcpyAsync(e->Fr.data(), e->frx, (size_t)nodes * 8, hipMemcpyDeviceToHost, ctx->stream));
+    BSAR_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = e;
+    e = nullptr;
+    return BSAR_OK;
+}
+
+}  // namespace
+
+int r1_tables(bsar_ctx* ctx, bsar_dataset* ds, int K, const double* gx, int nx, const double* gy, int ny,
+              const double* gz, int nz, R1Table* tab, const double** ftx, const double** frx)
+{
+    *tab = R1Table{};          // empty box: every lookup misses
+    tab->h = 1.0;
+    tab->ih = 1.0;
+    *ftx = *frx = nullptr;
+    if (K <= 0 || (int64_t)nx * ny * nz <= 0) return BSAR_OK;
+    std::lock_guard<std::mutex> lock(ds->tab_mu);
+    R1Cache* c = (R1Cache*)ds->r1tab;
+    if (!c) {
+        c = new R1Cache();
+        ds->r1tab = c;
+    }
+    if (!c->have_dem) {
+        c->X.resize(ds->dem.nx);
+        c->Y.resize(ds->dem.ny);
+        c->Z.resize((size_t)ds->dem.ny * ds->dem.zs);
+        BSAR_CHECK_HIP(ctx, hipMemcpy(c->X.data(), ds->dX, c->X.size() * 8, hipMemcpyDeviceToHost));
+        BSAR_CHECK_HIP(ctx, hipMemcpy(c->Y.data(), ds->dY, c->Y.size() * 8, hipMemcpyDeviceToHost));
+        BSAR_CHECK_HIP(ctx, hipMemcpy(c->Z.data(), ds->dZ, c->Z.size() * 8, hipMemcpyDeviceToHost));
+        c->have_dem = true;
+    }
+
+    // Refraction points of a 3x3x3 sample of the grid over the K pulses.
     auto pick = [](int m) {
         std::vector<int> v{0, m / 2, m - 1};
         std::sort(v.begin(), v.end());
@@ -121,69 +188,79 @@ static int ensure_tables(bsar_ctx* ctx, bsar_dataset* ds, const double* gx, int 
         for (int j : pick(ny))
             for (int i : pick(nx)) { pts.push_back(gx[i]); pts.push_back(gy[j]); pts.push_back(gz[k]); }
     const int64_t V = (int64_t)pts.size() / 3;
-    std::vector<double> qa((size_t)V * n * 3), qb((size_t)V * n * 3);
-    int st = bsar_slant_range_fermat(ctx, ds, pts.data(), V, qa.data(), qb.data());
-    if (st) return st;
+    const size_t m = (size_t)V * K;
+    std::vector<double> qa(m * 3), qb(m * 3);
+    {
+        struct B { void* p = nullptr; ~B() { if (p) (void)hipFree(p); } } bp, ba, bb;
+        BSAR_CHECK_HIP(ctx, hipMalloc(&bp.p, pts.size() * 8));
+        BSAR_CHECK_HIP(ctx, hipMalloc(&ba.p, m * 3 * 8));
+        BSAR_CHECK_HIP(ctx, hipMalloc(&bb.p, m * 3 * 8));
+        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(bp.p, pts.data(), pts.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        Traj tr = ds->traj;
+        tr.n = K;
+        BSAR_CHECK_HIP(ctx, launch_slant_range(ds->dem, tr, (const double*)bp.p, V, ds->p.n1, ds->p.n2,
+                                               (double*)ba.p, (double*)bb.p, ctx->stream));
+        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(qa.data(), ba.p, m * 3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(qb.data(), bb.p, m * 3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+        BSAR_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     double xa = INFINITY, xb = -INFINITY, ya = INFINITY, yb = -INFINITY;
     for (const auto* q : {&qa, &qb})
         for (size_t k = 0; k < q->size(); k += 3) {
             xa = std::min(xa, (*q)[k]); xb = std::max(xb, (*q)[k]);
             ya = std::min(ya, (*q)[k + 1]); yb = std::max(yb, (*q)[k + 1]);
         }
-    if (!(xb >= xa) || !(yb >= ya)) return BSAR_OK;   // non-finite geometry: exact path only
+    if (!(xb >= xa) || !(yb >= ya)) return BSAR_OK;   // non-finite geometry: exact sums only
     const double mx = 0.5 + 0.1 * (xb - xa), my = 0.5 + 0.1 * (yb - ya);
     xa -= mx; xb += mx; ya -= my; yb += my;
-
-    R1Cache* cur = (R1Cache*)ds->r1tab;
-    // DEM on host for the height range
-    std::vector<double> X(ds->dem.nx), Y(ds->dem.ny), Z((size_t)ds->dem.ny * ds->dem.zs);
-    BSAR_CHECK_HIP(ctx, hipMemcpy(X.data(), ds->dX, X.size() * 8, hipMemcpyDeviceToHost));
-    BSAR_CHECK_HIP(ctx, hipMemcpy(Y.data(), ds->dY, Y.size() * 8, hipMemcpyDeviceToHost));
-    BSAR_CHECK_HIP(ctx, hipMemcpy(Z.data(), ds->dZ, Z.size() * 8, hipMemcpyDeviceToHost));
     double za, zb;
-    dem_range_over_box(ds, X, Y, Z, xa, xb, ya, yb, &za, &zb);
+    dem_range_over_box(ds, *c, xa, xb, ya, yb, &za, &zb);
     za -= 0.01;
     zb += 0.01;
-    if (cur && xa >= cur->xa && xb <= cur->xb && ya >= cur->ya && yb <= cur->yb && za >= cur->za && zb <= cur->zb) {
-        *out = cur;
-        return BSAR_OK;
-    }
-    // grow the region to the union with the old one, then rebuild
-    if (cur) {
-        xa = std::min(xa, cur->xa); xb = std::max(xb, cur->xb);
-        ya = std::min(ya, cur->ya); yb = std::max(yb, cur->yb);
-        za = std::min(za, cur->za); zb = std::max(zb, cur->zb);
-        free_cache(cur);
-        ds->r1tab = nullptr;
-    }
-    R1Cache* c = new R1Cache();
-    c->xa = xa; c->xb = xb; c->ya = ya; c->yb = yb; c->za = za; c->zb = zb;
-    ds->r1tab = c;
-    double h = 0.25;
-    const int64_t max_nodes = int64_t(1) << 24;
-    for (int attempt = 0; attempt < 3; ++attempt, h *= 0.5) {
-        R1Table tb{};
-        tb.h = h;
-        tb.ih = 1.0 / h;
-        tb.x0 = xa - 1.5 * h;
-        tb.y0 = ya - 1.5 * h;
-        tb.z0 = za - 1.5 * h;
-        tb.nx = (int)std::ceil((xb - xa) / h) + 4;
-        tb.ny = (int)std::ceil((yb - ya) / h) + 4;
-        tb.nz = (int)std::ceil((zb - za) / h) + 4;
-        const int64_t nodes = (int64_t)tb.nx * tb.ny * tb.nz;
-        if (nodes > max_nodes) break;
-        if (c->ftx) { (void)hipFree(c->ftx); c->ftx = nullptr; }
-        if (c->frx) { (void)hipFree(c->frx); c->frx = nullptr; }
-        BSAR_CHECK_HIP(ctx, hipMalloc((void**)&c->ftx, (size_t)nodes * 8));
-        BSAR_CHECK_HIP(ctx, hipMalloc((void**)&c->frx, (size_t)nodes * 8));
-        BSAR_CHECK_HIP(ctx, launch_r1_table(tb, ds->traj.tx, ds->traj.ty, ds->traj.tz, n, c->ftx, ctx->stream));
-        BSAR_CHECK_HIP(ctx, launch_r1_table(tb, ds->traj.rx, ds->traj.ry, ds->traj.rz, n, c->frx, ctx->stream));
-        // Self-check against exact sums at pseudo-random points of the region.
-        std::vector<double> Ft((size_t)nodes), Fr((size_t)nodes);
-        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(Ft.data(), c->ftx, (size_t)nodes * 8, hipMemcpyDeviceToHost, ctx->stream));
-        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(Fr.data(), c->frx, (size_t)nodes * 8, hipMemcpyDeviceToHost, ctx->stream));
-        BSAR_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double lim = 1e9;   // lattice indices stay well inside int
+    for (double h : {0.25, 0.125, 0.0625}) {
+        const double ih = 1.0 / h;
+        const double bx[2] = {std::floor(xa * ih), std::ceil(xb * ih)};
+        const double by[2] = {std::floor(ya * ih), std::ceil(yb * ih)};
+        const double bz[2] = {std::floor(za * ih), std::ceil(zb * ih)};
+        bool ok = true;
+        for (double v : {bx[0], bx[1], by[0], by[1], bz[0], bz[1]}) ok = ok && std::fabs(v) < lim;
+        if (!ok) break;
+        // nodes [lo - 1, hi + 1] carry the stencils of every u in [lo, hi)
+        const int ax = (int)bx[0] - 1, ay = (int)by[0] - 1, az = (int)bz[0] - 1;
+        const int nxn = (int)(bx[1] - bx[0]) + 3, nyn = (int)(by[1] - by[0]) + 3, nzn = (int)(bz[1] - bz[0]) + 3;
+        if ((int64_t)nxn * nyn * nzn > kMaxNodes) break;
+        R1Entry* e = nullptr;
+        R1Entry* newest = nullptr;
+        for (R1Entry* x : c->entries) {
+            if (x->K != K || x->geo.h != h) continue;
+            newest = x;
+            if (covers(x->geo, ax, ay, az, nxn, nyn, nzn)) e = x;
+        }
+        if (!e) {
+            R1Table geo{};
+            geo.h = h;
+            geo.ih = ih;
+            geo.ax = ax; geo.ay = ay; geo.az = az;
+            geo.nx = nxn; geo.ny = nyn; geo.nz = nzn;
+            if (newest) {   // grow to the union with the newest table of this (K, h)
+                const R1Table& o = newest->geo;
+                R1Table u = geo;
+                u.ax = std::min(ax, o.ax); u.ay = std::min(ay, o.ay); u.az = std::min(az, o.az);
+                u.nx = std::max(ax + nxn, o.ax + o.nx) - u.ax;
+                u.ny = std::max(ay + nyn, o.ay + o.ny) - u.ay;
+                u.nz = std::max(az + nzn, o.az + o.nz) - u.az;
+                if ((int64_t)u.nx * u.ny * u.nz <= kMaxNodes) geo = u;
+            }
+            int st = build_entry(ctx, ds, K, geo, &e);
+            if (st) return st;
+            c->entries.push_back(e);
+        }
+        R1Table tb = e->geo;
+        tb.lo[0] = bx[0]; tb.hi[0] = bx[1];
+        tb.lo[1] = by[0]; tb.hi[1] = by[1];
+        tb.lo[2] = bz[0]; tb.hi[2] = bz[1];
+        // Self-check against exact sums at pseudo-random points of the box.
         uint64_t sst = 0x9E3779B97F4A7C15ull;
         auto u01 = [&]() {
             sst += 0x9E3779B97F4A7C15ull;
@@ -197,29 +274,38 @@ static int ensure_tables(bsar_ctx* ctx, bsar_dataset* ds, const double* gx, int 
         for (int k = 0; k < 96; ++k) {
             const double x = xa + (xb - xa) * u01(), y = ya + (yb - ya) * u01(), z = za + (zb - za) * u01();
             double vt, vr;
-            if (host_lookup(tb, Ft, x, y, z, &vt)) maxerr = std::max(maxerr, std::fabs(vt - host_r1(ds->htx, n, x, y, z)));
-            if (host_lookup(tb, Fr, x, y, z, &vr)) maxerr = std::max(maxerr, std::fabs(vr - host_r1(ds->hrx, n, x, y, z)));
+            if (host_lookup(tb, e->Ft, x, y, z, &vt)) maxerr = std::max(maxerr, std::fabs(vt - host_r1(ds->htx, K, x, y, z)));
+            if (host_lookup(tb, e->Fr, x, y, z, &vr)) maxerr = std::max(maxerr, std::fabs(vr - host_r1(ds->hrx, K, x, y, z)));
         }
-        c->tab = tb;
         // 1e-7 m of mean slant range is 2*pi/lambda*1e-7 rad of phase
         // (2e-6 rad at lambda = 0.3 m): far below the -80 dB image budget.
-        if (maxerr <= 1e-7) { c->usable = true; break; }
+        if (maxerr <= 1e-7) {
+            *tab = tb;
+            *ftx = e->ftx;
+            *frx = e->frx;
+            return BSAR_OK;
+        }
     }
-    if (!c->usable) {
-        c->tab.nx = c->tab.ny = c->tab.nz = 0;   // every lookup misses: exact sums
-    }
-    *out = c;
-    return BSAR_OK;
+    return BSAR_OK;   // no spacing passed: exact sums
 }
 
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
-__global__ void k_f4_to_d2_inplace(void* p, int64_t n)
+// out[v] = (double)sum - (double)comp; out may alias acc.
+__global__ void k_f4_to_d2(const float4* acc, int64_t n, double2* out)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const float4 x = reinterpret_cast<float4*>(p)[k];
-    reinterpret_cast<double2*>(p)[k] = make_double2((double)x.x - (double)x.z, (double)x.y - (double)x.w);
+    const float4 x = acc[k];
+    out[k] = make_double2((double)x.x - (double)x.z, (double)x.y - (double)x.w);
+}
+
+hipError_t finish_mixed(const float4* acc, int groups, int64_t nvox, double2* out, hipStream_t s)
+{
+    if (nvox <= 0) return hipSuccess;
+    if (groups > 1) return launch_group_sum_mixed(acc, groups, nvox, nvox, out, s);
+    hipLaunchKernelGGL(k_f4_to_d2, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, s, acc, nvox, out);
+    return hipGetLastError();
 }
 
 int backproject_mixed(bsar_ctx* ctx, const bsar_dataset* cds, const double* gx, int nx, const double* gy, int ny,
@@ -228,13 +314,12 @@ int backproject_mixed(bsar_ctx* ctx, const bsar_dataset* cds, const double* gx, 
     bsar_dataset* ds = const_cast<bsar_dataset*>(cds);   // the table cache is mutable state
     const int64_t V = (int64_t)nx * ny * nz;
     const int n = ds->n;
-    R1Cache* cache = nullptr;
-    if (V > 0 && n > 0) {
-        int st = ensure_tables(ctx, ds, gx, nx, gy, ny, gz, nz, &cache);
+    MixedArgs m{};
+    {
+        int st = r1_tables(ctx, ds, n, gx, nx, gy, ny, gz, nz, &m.tab, &m.ftx, &m.frx);
         if (st) return st;
     }
-    const double cost = 1418.0 * 60.0 + 600.0;
-    BpPlan pl = plan_bp(V, n, cost);
+    BpPlan pl = plan_bp(V, n, bp_cost_mixed());
     struct Buf { void* p = nullptr; ~Buf() { if (p) (void)hipFree(p); } } axes, accb, cnt;
     BSAR_CHECK_HIP(ctx, hipMalloc(&axes.p, (size_t)(nx + ny + nz + 1) * sizeof(double)));
     double* dgx = (double*)axes.p;
@@ -251,37 +336,7 @@ int backproject_mixed(bsar_ctx* ctx, const bsar_dataset* cds, const double* gx, 
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, ctx->stream, (unsigned long long*)cnt.p);
     BSAR_CHECK_HIP(ctx, hipGetLastError());
 
-    BpArgs a{};
-    a.dem = ds->dem;
-    a.traj = ds->traj;
-    a.rc = ds->rc;
-    a.rc32 = ds->rc32;
-    a.ndata = ds->ndata;
-    a.S4 = ds->S4;
-    a.s4m1 = (double)(ds->S4 - 1);
-    a.n1 = ds->p.n1;
-    a.n2 = ds->p.n2;
-    a.inv_c = 1.0 / ds->p.c;
-    a.fs4 = ds->p.fs * ds->p.upsample;
-    a.kphi = 2.0 * M_PI / (ds->p.c / ds->p.fc);
-    a.gx = dgx;
-    a.gy = dgx + nx;
-    a.gz = dgx + nx + ny;
-    a.nx = nx; a.ny = ny; a.nz = nz;
-    a.vox0 = 0;
-    a.nvox = V;
-    a.nvox_pad = pl.nvox_pad;
-    a.groups = pl.groups;
-    a.glen = pl.glen;
-    a.wlen = pl.wlen;
-    a.acc = nullptr;
-    a.acc_stride = V;
-    MixedArgs m{};
-    if (cache) {
-        m.tab = cache->tab;
-        m.ftx = cache->ftx;
-        m.frx = cache->frx;
-    }
+    BpArgs a = make_bp_args(ds, n, dgx, nx, ny, nz, pl);
     m.acc = acc;
     m.exact_count = (unsigned long long*)cnt.p;
     m.two_pi = 2.0 * M_PI;
@@ -297,13 +352,7 @@ int backproject_mixed(bsar_ctx* ctx, const bsar_dataset* cds, const double* gx, 
         }
     }
     if (V > 0) {
-        if (pl.groups > 1) {
-            BSAR_CHECK_HIP(ctx, launch_group_sum_mixed(acc, pl.groups, V, V, d_out, ctx->stream));
-        } else {
-            hipLaunchKernelGGL(k_f4_to_d2_inplace, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, ctx->stream,
-                               (void*)d_out, V);
-            BSAR_CHECK_HIP(ctx, hipGetLastError());
-        }
+        BSAR_CHECK_HIP(ctx, finish_mixed(acc, pl.groups, V, d_out, ctx->stream));
         ++launches;
     }
     BSAR_CHECK_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));

@@ -196,6 +196,79 @@ BSAR_API int bsar_write_matrix2d(const char* path, const double* data, uint64_t 
  * delay/phase, lerp and accumulate (48): 65,406 + 20 N. */
 BSAR_API double bsar_flops_per_voxel_pulse(int n_pulses);
 
+/* ---- serving loop --------------------------------------------------------
+ *
+ * No counterpart in the reference, whose driver images one grid per process
+ * run (proc_test_script.cpp:main).  A server owns one device-side scheduler:
+ * a host thread admits submitted jobs in arrival order while slots and the
+ * HBM cap allow, and runs scheduling rounds in which the admitted jobs'
+ * (pulse group, 256-voxel tile, pulse window) work items are pulled by
+ * persistent kernels from one list, so images of different jobs run side by
+ * side.  A job's image is bit-identical to bsar_backproject of the same grid
+ * over a dataset holding the job's first K pulses, whatever else runs. */
+
+typedef struct bsar_server bsar_server;
+
+typedef struct {
+    int max_jobs;            /* jobs admitted at once; 0 = 256 */
+    int reserved0;
+    int64_t hbm_cap_bytes;   /* cap on the admitted jobs' device working memory; 0 = 200e9 */
+    double round_seconds;    /* target device time of one scheduling round; 0 = 0.04 */
+    double reserved[4];
+} bsar_server_config;
+
+typedef struct {
+    const bsar_dataset* ds;  /* resident dataset on the server's device, range data loaded */
+    int pulses;              /* the job images the first K pulses of ds; 0 = all */
+    int precision;           /* BSAR_FP64 or BSAR_MIXED */
+    const double* gx; int nx;
+    const double* gy; int ny;
+    const double* gz; int nz;
+    double* out;             /* host image [nz][ny][nx] complex, written before the job is done */
+} bsar_job_desc;
+
+typedef enum { BSAR_JOB_QUEUED = 0, BSAR_JOB_RUNNING = 1, BSAR_JOB_DONE = 2, BSAR_JOB_FAILED = 3 } bsar_job_state;
+
+typedef struct {
+    int64_t id;
+    int state;               /* bsar_job_state */
+    int status;              /* BSAR_OK, or why the job failed */
+    double t_submit, t_admit, t_done;   /* seconds on the server clock (bsar_server_clock) */
+    double voxel_pulses;
+    double r1_exact_frac;    /* mixed: share of R1 means summed exactly */
+    int64_t footprint_bytes; /* device working memory held while admitted */
+} bsar_job_info;
+
+typedef struct {
+    int64_t submitted, completed, failed, rejected;
+    int64_t rounds, items;
+    double device_seconds;   /* summed device time of the rounds */
+    double voxel_pulses;     /* work of the completed jobs */
+    int64_t hbm_in_use, hbm_peak;
+    int active, queued;
+} bsar_server_stats;
+
+/* Starts the scheduler thread on `device`.  cfg may be null (defaults). */
+BSAR_API int bsar_server_create(int device, const bsar_server_config* cfg, bsar_server** out);
+/* Stops after the admitted and queued jobs finish; frees the server. */
+BSAR_API void bsar_server_destroy(bsar_server* s);
+BSAR_API const char* bsar_server_last_error(const bsar_server* s);
+/* Seconds since the server started (steady clock). */
+BSAR_API double bsar_server_clock(const bsar_server* s);
+/* Queues a job (grid axes are copied; ds and out must stay valid until the
+ * job is collected).  BSAR_ERR_CAPACITY when its working memory alone exceeds
+ * the HBM cap. */
+BSAR_API int bsar_server_submit(bsar_server* s, const bsar_job_desc* job, int64_t* id);
+/* Non-blocking state of a job. */
+BSAR_API int bsar_server_poll(bsar_server* s, int64_t id, bsar_job_info* info);
+/* Blocks until the job is done or failed (timeout_s < 0: no limit), then
+ * collects it (its id becomes unknown).  Returns the job's status, or
+ * BSAR_ERR_STATE on timeout. */
+BSAR_API int bsar_server_wait(bsar_server* s, int64_t id, double timeout_s, bsar_job_info* info);
+/* Blocks until no job is queued or running. */
+BSAR_API int bsar_server_drain(bsar_server* s);
+BSAR_API int bsar_server_get_stats(bsar_server* s, bsar_server_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

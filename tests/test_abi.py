@@ -52,6 +52,32 @@ def test_context_without_device_fails_cleanly(bsar):
     assert L.bsar_ctx_create(-1, C.byref(h)) in (-1, -2)
 
 
+def test_server_without_device_fails_cleanly(bsar):
+    import ctypes as C
+    L = bsar.lib()
+    h = C.c_void_p()
+    st = L.bsar_server_create(0, None, C.byref(h))
+    if st == 0:
+        L.bsar_server_destroy(h)
+        pytest.skip("a GPU is present")
+    assert st == -2 and not h.value
+    L.bsar_server_destroy(None)                      # null is a no-op
+    assert L.bsar_server_submit(None, None, None) == -1
+    assert L.bsar_server_wait(None, 1, 0.0, None) == -1
+    assert L.bsar_server_drain(None) == -1
+    assert L.bsar_server_last_error(None) == b"null server"
+
+
+def test_server_structs_match_header(bsar):
+    """ctypes mirrors of bsar_server_config / bsar_job_desc / bsar_job_info /
+    bsar_server_stats have the C layouts (x86-64 SysV)."""
+    import ctypes as C
+    assert C.sizeof(bsar.ServerConfigC) == 4 + 4 + 8 + 8 + 4 * 8
+    assert C.sizeof(bsar.JobDescC) == 8 + 4 + 4 + 3 * 16 + 8
+    assert C.sizeof(bsar.JobInfoC) == 8 + 4 + 4 + 5 * 8 + 8
+    assert C.sizeof(bsar.ServerStatsC) == 6 * 8 + 2 * 8 + 2 * 8 + 2 * 4
+
+
 def test_chirp_matches_reference(bsar, golden):
     for tag in ("refc", "cfg"):
         f0, f1, T, fs = golden[f"chirp_{tag}_args"]
