@@ -5,6 +5,7 @@ image against the FP64 image in dB and the reference's OpenMP CPU path timed
 on the same host.
 
     python bench.py [--steps K] [--warmup W] [--workload large|small|...]
+    python bench.py --workload serve [--jobs 256] [--rate 40] [--no-fp64]
 
 One step = one full mixed-precision image of the workload with the
 range-compressed data already resident in HBM.  Prints one JSON line on
@@ -163,6 +164,88 @@ def cpu_baseline(sc, root, target_s: float = 20.0) -> dict | None:
             "sample": f"{vox} voxels x {sc.n} pulses, C restatement with the reflection search"}
 
 
+def run_serve(args, pkg, scene) -> dict:
+    """Serving stream: jobs arrive (Poisson) and run through one bsar_server
+    with continuous batching; the same stream runs once in mixed precision and
+    once in plain FP64.  Latency = submission -> image in the caller's host
+    buffer (server clock); throughput = completed voxel-pulses / makespan."""
+    import threading
+    sc = scene.make_scene("serve")
+    jobs = scene.serve_jobs(args.jobs, args.rate)
+    total_vp = sum(j["vp"] for j in jobs)
+    log(f"serving stream: {len(jobs)} jobs, Poisson {args.rate}/s over {jobs[-1]['t']:.2f} s, "
+        f"{total_vp:.3e} voxel-pulses; dataset {sc.n} pulses x {sc.S} samples")
+    samples = [k for k, j in enumerate(jobs) if np.prod(j["shape"]) <= 64 ** 3][:6]
+    forms = [("mixed", pkg.MIXED)] + ([] if args.no_fp64 else [("fp64", pkg.FP64)])
+    res, imgs = {}, {}
+    with pkg.Context(0) as ctx:
+        t0 = time.time()
+        root = scene.simulate(ctx, sc)
+        log(f"forward model {time.time() - t0:.1f} s")
+        ds = ctx.dataset(pkg.Params(sc.fc, sc.c, sc.fs, sc.n1, sc.n2, sc.upsample), pkg.Dem(*sc.dem), sc.tx, sc.rx)
+        ds.load_rootdata(root)
+        del root
+        log(f"dataset resident: {ds.device_bytes() / 1e9:.2f} GB")
+        for name, prec in forms:
+            with pkg.Server(round_seconds=args.round_ms * 1e-3) as srv:
+                ids = [None] * len(jobs)
+                t_start = time.perf_counter()
+
+                def feed():
+                    for k, j in enumerate(jobs):
+                        dt = t_start + j["t"] - time.perf_counter()
+                        if dt > 0:
+                            time.sleep(dt)
+                        ids[k] = srv.submit(ds, j["gx"], j["gy"], j["gz"], pulses=j["K"], precision=prec)
+
+                th = threading.Thread(target=feed)
+                th.start()
+                infos = []
+                for k in range(len(jobs)):
+                    while ids[k] is None:
+                        time.sleep(0.001)
+                    img, info = srv.wait(ids[k])
+                    infos.append(info)
+                    if k in samples:
+                        imgs[(name, k)] = img
+                    if (k + 1) % 32 == 0 or k + 1 == len(jobs):
+                        log(f"{name}: {k + 1}/{len(jobs)} jobs collected, {time.perf_counter() - t_start:.1f} s")
+                th.join()
+                st = srv.stats()
+            t_sub = min(i["t_submit"] for i in infos)
+            t_end = max(i["t_done"] for i in infos)
+            span = t_end - t_sub
+            lat = np.array([i["latency"] for i in infos])
+            res[name] = {"value": round(total_vp / span, 1), "makespan_s": round(span, 3),
+                         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 1),
+                                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 1),
+                                        "mean": round(float(lat.mean()) * 1e3, 1)},
+                         "device_busy": round(st["device_seconds"] / span, 3), "rounds": st["rounds"],
+                         "items": st["items"], "hbm_peak_gb": round(st["hbm_peak"] / 1e9, 3),
+                         "r1_exact_frac_max": max(i["r1_exact_frac"] for i in infos) if name == "mixed" else None}
+            log(f"{name}: {res[name]['value']:.3e} voxel-pulse/s, p50 {res[name]['latency_ms']['p50']} ms, "
+                f"p99 {res[name]['latency_ms']['p99']} ms, device busy {res[name]['device_busy']}")
+        ds.close()
+    m = res["mixed"]
+    out = {"metric": "serving stream: aggregate voxel-pulse BP/s (mixed precision) with p50/p99 job latency",
+           "value": m["value"], "unit": "voxel-pulse/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": round(m["makespan_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic (seeded scene and job stream)",
+           "config": {"workload": f"serve: {len(jobs)} jobs, Poisson {args.rate}/s, grids {scene.SERVE_GRIDS}, "
+                                  f"pulses {scene.SERVE_PULSES} (first K of {sc.n} x {sc.S} samples x4)",
+                      "jobs": len(jobs), "rate": args.rate, "voxel_pulses": total_vp,
+                      "round_ms": args.round_ms},
+           "latency_ms": m["latency_ms"], "mixed": m}
+    if "fp64" in res:
+        out["fp64"] = res["fp64"]
+        errs = [float(np.abs(imgs[("mixed", k)] - imgs[("fp64", k)]).max() / np.abs(imgs[("fp64", k)]).max())
+                for k in samples]
+        out["mixed_vs_fp64"] = {"throughput_ratio": round(m["value"] / res["fp64"]["value"], 3),
+                                "peak_err_db_max": round(20 * np.log10(max(max(errs), 1e-300)), 2),
+                                "jobs_compared": len(samples)}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +254,9 @@ def main() -> int:
     ap.add_argument("--workload", default="large")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--jobs", type=int, default=256, help="serve: jobs in the stream")
+    ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate, jobs/s")
+    ap.add_argument("--round-ms", type=float, default=40.0, help="serve: target device time per round")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -179,6 +265,9 @@ def main() -> int:
 
     pkg = load("bsar_amd", PKG_DIR / "__init__.py", pkg=True)
     scene = load("bsar_amd.scene", PKG_DIR / "scene.py")
+    if args.workload == "serve":
+        print(json.dumps(run_serve(args, pkg, scene)), flush=True)
+        return 0
     sc = scene.make_scene(args.workload)
     V, N = sc.voxels, sc.n
     log(f"workload {sc.name}: {len(sc.gx)}x{len(sc.gy)}x{len(sc.gz)} voxels x {N} pulses, {sc.S} samples")

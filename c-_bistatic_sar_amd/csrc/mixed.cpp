@@ -9,8 +9,10 @@
 // its box every table of that (K, h) answers identically, outside it the
 // kernel sums exactly.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "api_internal.h"
@@ -32,6 +34,9 @@ struct R1Cache {
     std::vector<R1Entry*> entries;    // immutable once built, newest last
     bool have_dem = false;
     std::vector<double> X, Y, Z;      // host DEM for height ranges
+    // refraction-point extent (xa, xb, ya, yb) of a grid sample over K pulses,
+    // keyed by {K, sample points}: repeated grids need no device work
+    std::map<std::vector<double>, std::array<double, 4>> extents;
 };
 
 void free_cache(R1Cache* c)
@@ -150,6 +155,38 @@ int build_entry(bsar_ctx* ctx, const bsar_dataset* ds, int K, const R1Table& geo
     return BSAR_OK;
 }
 
+// Horizontal extent of the refraction points of the sample points pts over
+// the first K pulses (outbound and return legs).
+int sample_extent(bsar_ctx* ctx, const bsar_dataset* ds, int K, const std::vector<double>& pts,
+                  std::array<double, 4>& ext)
+{
+    const int64_t V = (int64_t)pts.size() / 3;
+    const size_t m = (size_t)V * K;
+    std::vector<double> qa(m * 3), qb(m * 3);
+    {
+        struct B { void* p = nullptr; ~B() { if (p) (void)hipFree(p); } } bp, ba, bb;
+        BSAR_CHECK_HIP(ctx, hipMalloc(&bp.p, pts.size() * 8));
+        BSAR_CHECK_HIP(ctx, hipMalloc(&ba.p, m * 3 * 8));
+        BSAR_CHECK_HIP(ctx, hipMalloc(&bb.p, m * 3 * 8));
+        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(bp.p, pts.data(), pts.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        Traj tr = ds->traj;
+        tr.n = K;
+        BSAR_CHECK_HIP(ctx, launch_slant_range(ds->dem, tr, (const double*)bp.p, V, ds->p.n1, ds->p.n2,
+                                               (double*)ba.p, (double*)bb.p, ctx->stream));
+        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(qa.data(), ba.p, m * 3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(qb.data(), bb.p, m * 3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+        BSAR_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    double xa = INFINITY, xb = -INFINITY, ya = INFINITY, yb = -INFINITY;
+    for (const auto* q : {&qa, &qb})
+        for (size_t k = 0; k < q->size(); k += 3) {
+            xa = std::min(xa, (*q)[k]); xb = std::max(xb, (*q)[k]);
+            ya = std::min(ya, (*q)[k + 1]); yb = std::max(yb, (*q)[k + 1]);
+        }
+    ext = {xa, xb, ya, yb};
+    return BSAR_OK;
+}
+
 }  // namespace
 
 int r1_tables(bsar_ctx* ctx, bsar_dataset* ds, int K, const double* gx, int nx, const double* gy, int ny,
@@ -187,29 +224,18 @@ int r1_tables(bsar_ctx* ctx, bsar_dataset* ds, int K, const double* gx, int nx, 
     for (int k : pick(nz))
         for (int j : pick(ny))
             for (int i : pick(nx)) { pts.push_back(gx[i]); pts.push_back(gy[j]); pts.push_back(gz[k]); }
-    const int64_t V = (int64_t)pts.size() / 3;
-    const size_t m = (size_t)V * K;
-    std::vector<double> qa(m * 3), qb(m * 3);
-    {
-        struct B { void* p = nullptr; ~B() { if (p) (void)hipFree(p); } } bp, ba, bb;
-        BSAR_CHECK_HIP(ctx, hipMalloc(&bp.p, pts.size() * 8));
-        BSAR_CHECK_HIP(ctx, hipMalloc(&ba.p, m * 3 * 8));
-        BSAR_CHECK_HIP(ctx, hipMalloc(&bb.p, m * 3 * 8));
-        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(bp.p, pts.data(), pts.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-        Traj tr = ds->traj;
-        tr.n = K;
-        BSAR_CHECK_HIP(ctx, launch_slant_range(ds->dem, tr, (const double*)bp.p, V, ds->p.n1, ds->p.n2,
-                                               (double*)ba.p, (double*)bb.p, ctx->stream));
-        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(qa.data(), ba.p, m * 3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-        BSAR_CHECK_HIP(ctx, hipMemcpyAsync(qb.data(), bb.p, m * 3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-        BSAR_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<double> key;
+    key.reserve(pts.size() + 1);
+    key.push_back((double)K);
+    key.insert(key.end(), pts.begin(), pts.end());
+    auto hit = c->extents.find(key);
+    if (hit == c->extents.end()) {
+        std::array<double, 4> ext;
+        int st = sample_extent(ctx, ds, K, pts, ext);
+        if (st) return st;
+        hit = c->extents.emplace(key, ext).first;
     }
-    double xa = INFINITY, xb = -INFINITY, ya = INFINITY, yb = -INFINITY;
-    for (const auto* q : {&qa, &qb})
-        for (size_t k = 0; k < q->size(); k += 3) {
-            xa = std::min(xa, (*q)[k]); xb = std::max(xb, (*q)[k]);
-            ya = std::min(ya, (*q)[k + 1]); yb = std::max(yb, (*q)[k + 1]);
-        }
+    double xa = hit->second[0], xb = hit->second[1], ya = hit->second[2], yb = hit->second[3];
     if (!(xb >= xa) || !(yb >= ya)) return BSAR_OK;   // non-finite geometry: exact sums only
     const double mx = 0.5 + 0.1 * (xb - xa), my = 0.5 + 0.1 * (yb - ya);
     xa -= mx; xb += mx; ya -= my; yb += my;

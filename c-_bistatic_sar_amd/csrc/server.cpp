@@ -103,6 +103,8 @@ struct Round {
     int64_t items = 0;
 };
 
+constexpr size_t kAxesMax = 3 * 4096 + 1;   // doubles of staged axes per slot
+
 size_t axes_bytes(const Job* j) { return (size_t)(j->nx + j->ny + j->nz + 1) * sizeof(double); }
 size_t acc_bytes(const Job* j) { return ((size_t)j->pl.groups * j->V * 16 + 255) / 256 * 256; }
 
@@ -132,6 +134,7 @@ struct bsar_server {
     std::vector<Job*> active;
     std::vector<int> free_slots;
     SrvJobDev* h_jobs = nullptr;        // pinned [max_jobs]
+    double* h_axes = nullptr;           // pinned [max_jobs][kAxesMax]: grid axes staged for upload
     SrvJobDev* d_jobs = nullptr;
     unsigned long long* d_counts = nullptr;
     SrvList* h_list[2] = {nullptr, nullptr};
@@ -250,8 +253,15 @@ int setup(bsar_server* s, Job* j)
     const size_t ab = acc_bytes(j);
     BSAR_CHECK_HIP(c, s->pool.get(ab + axes_bytes(j), &j->dmem, &j->dcap));
     double* dg = (double*)((char*)j->dmem + ab);
-    BSAR_CHECK_HIP(c, hipMemcpyAsync(dg, j->axes.data(), j->axes.size() * 8, hipMemcpyHostToDevice, c->stream));
-    BSAR_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    if (j->axes.size() <= kAxesMax) {
+        // through the slot's pinned staging area, in round order: no host wait
+        double* h = s->h_axes + (size_t)j->slot * kAxesMax;
+        std::memcpy(h, j->axes.data(), j->axes.size() * 8);
+        BSAR_CHECK_HIP(c, hipMemcpyAsync(dg, h, j->axes.size() * 8, hipMemcpyHostToDevice, s->cs));
+    } else {
+        BSAR_CHECK_HIP(c, hipMemcpyAsync(dg, j->axes.data(), j->axes.size() * 8, hipMemcpyHostToDevice, c->stream));
+        BSAR_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    }
     const bool mixed = j->precision == BSAR_MIXED;
     const int64_t nacc = (int64_t)j->pl.groups * j->V;
     if (mixed) BSAR_CHECK_HIP(c, fill_zero4((float4*)j->dmem, nacc, s->cs));
@@ -529,6 +539,7 @@ void destroy_server(bsar_server* s)
         if (s->ev_end[b]) (void)hipEventDestroy(s->ev_end[b]);
     }
     if (s->h_jobs) (void)hipHostFree(s->h_jobs);
+    if (s->h_axes) (void)hipHostFree(s->h_axes);
     if (s->d_jobs) (void)hipFree(s->d_jobs);
     if (s->d_counts) (void)hipFree(s->d_counts);
     s->pool.trim();
@@ -571,6 +582,9 @@ int bsar_server_create(int device, const bsar_server_config* cfg, bsar_server** 
     if ((e = hipHostMalloc((void**)&s->h_jobs, (size_t)M * sizeof(SrvJobDev), hipHostMallocDefault)) != hipSuccess)
         return fail(e);
     std::memset((void*)s->h_jobs, 0, (size_t)M * sizeof(SrvJobDev));
+    if ((e = hipHostMalloc((void**)&s->h_axes, (size_t)M * kAxesMax * sizeof(double), hipHostMallocDefault)) !=
+        hipSuccess)
+        return fail(e);
     if ((e = hipMalloc((void**)&s->d_jobs, (size_t)M * sizeof(SrvJobDev))) != hipSuccess) return fail(e);
     if ((e = hipMalloc((void**)&s->d_counts, (size_t)M * sizeof(unsigned long long))) != hipSuccess) return fail(e);
     if ((e = alloc_lists(s, kSrvListCap)) != hipSuccess) return fail(e);

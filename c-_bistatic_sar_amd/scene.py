@@ -134,6 +134,8 @@ def make_scene(name: str, seed: int | None = None, n_pulses: int | None = None, 
       large   201x201x101 grid, 2048 pulses, 8192 samples, 16 scatterers + clutter layer
       rough   128^3 grid 4x4x4 m buried 3-7 m under fBm terrain (167x100 DEM), 1024 pulses
       wide    512x512x64 grid, 4096 pulses, 16384 samples, 64 scatterers, B = 500 MHz
+      serve   resident 4096-pulse x 8192-sample dataset for the serving stream
+              (serve_jobs: grids and pulse counts of the jobs), 16 scatterers
     Every one: fc = 1 GHz, T = 1 us, fs = 800 MHz, c = 3e8, n1 = 1, n2 = sqrt(eps_r).
     """
     seeds = dict(small=1001, large=1002, rough=1003, wide=1004, serve=1005)
@@ -179,6 +181,13 @@ def make_scene(name: str, seed: int | None = None, n_pulses: int | None = None, 
         scat = _scatterers(rng, 64)
         rcs = np.ones(64, np.complex128)
         snr = None
+    elif name == "serve":
+        N, S, bw, er = 4096, 8192, 300e6, 6.0
+        gx, gy, gz = _grid(41, 41, 21)
+        dem = dem_bumps(rng)
+        scat = _scatterers(rng, 16)
+        rcs = np.ones(16, np.complex128)
+        snr = None
     else:
         raise ValueError(f"unknown workload {name!r}")
     if n_pulses is not None:
@@ -196,6 +205,27 @@ def make_scene(name: str, seed: int | None = None, n_pulses: int | None = None, 
         rx = spiral(N, phase=np.pi / 2)
     return Scene(name=name, fc=fc, c=c, fs=fs, bw=bw, T=T, n1=1.0, n2=float(np.sqrt(er)), S=S, tx=tx, rx=rx,
                  dem=dem, gx=gx, gy=gy, gz=gz, scat=scat, rcs=rcs, snr_db=snr, seed=seed)
+
+
+SERVE_GRIDS = [(16, 16, 16), (32, 32, 32), (64, 64, 64), (96, 96, 48), (128, 128, 64)]
+SERVE_PULSES = [128, 512, 1024, 2048]
+
+
+def serve_jobs(count: int = 256, rate: float = 40.0, seed: int = 1005) -> list[dict]:
+    """Serving stream: Poisson arrivals at `rate` jobs/s; each job images a grid
+    drawn uniformly from SERVE_GRIDS over the common box with the first K
+    pulses, K uniform in SERVE_PULSES."""
+    rng = np.random.default_rng(seed + 1)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    gi = rng.integers(0, len(SERVE_GRIDS), count)
+    ki = rng.integers(0, len(SERVE_PULSES), count)
+    jobs = []
+    for k in range(count):
+        shape = SERVE_GRIDS[gi[k]]
+        gx, gy, gz = _grid(*shape)
+        jobs.append(dict(t=float(t[k]), shape=shape, K=SERVE_PULSES[ki[k]], gx=gx, gy=gy, gz=gz,
+                         vp=float(np.prod(shape)) * SERVE_PULSES[ki[k]]))
+    return jobs
 
 
 def reference_signal(chirp: np.ndarray, S: int) -> np.ndarray:
