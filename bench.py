@@ -81,15 +81,20 @@ def roofline(rep: dict, n: int, form: str, traffic) -> dict:
 
 
 def traffic_from_profiles(kernel: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
-    p = ROOT / "profiles" / "traffic_bp_large.json"
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE passes, calibrated against a copy
+    kernel of known size); None when it is absent."""
+    p = ROOT / "profiles" / "traffic_bp_large.md"
     if not p.exists():
         return None
-    try:
-        d = json.loads(p.read_text())
-        return d.get(kernel, {}).get("hbm_bytes_per_launch")
-    except Exception:
-        return None
+    for line in p.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == kernel:
+            try:
+                return int(cells[4])
+            except ValueError:
+                return None
+    return None
 
 
 def write_reference_inputs(proc: Path, sc, root: np.ndarray, gx, gy, gz) -> None:
