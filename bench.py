@@ -206,15 +206,24 @@ def run_serve(args, pkg, scene) -> dict:
                 th = threading.Thread(target=feed)
                 th.start()
                 infos = []
-                for k in range(len(jobs)):
-                    while ids[k] is None:
-                        time.sleep(0.001)
-                    img, info = srv.wait(ids[k])
+                while len(infos) < len(jobs):
+                    try:
+                        jid, img, info = srv.wait_any(timeout=1.0)
+                    except TimeoutError:
+                        continue
+                    except pkg.BsarError as e:
+                        if e.status == -1 and len(infos) < len(jobs):   # nothing submitted yet
+                            time.sleep(0.01)
+                            continue
+                        raise
                     infos.append(info)
+                    k = ids.index(jid)
                     if k in samples:
                         imgs[(name, k)] = img
-                    if (k + 1) % 32 == 0 or k + 1 == len(jobs):
-                        log(f"{name}: {k + 1}/{len(jobs)} jobs collected, {time.perf_counter() - t_start:.1f} s")
+                    j = jobs[k]
+                    log(f"{name}: job {k} ({'x'.join(map(str, j['shape']))} x {j['K']}) done in "
+                        f"{info['latency']:.2f} s; {len(infos)}/{len(jobs)} done at "
+                        f"{time.perf_counter() - t_start:.1f} s")
                 th.join()
                 st = srv.stats()
             t_sub = min(i["t_submit"] for i in infos)

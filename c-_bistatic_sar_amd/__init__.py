@@ -114,6 +114,7 @@ SIGNATURES = {
     "bsar_server_submit": (C.c_int, [C.c_void_p, C.POINTER(JobDescC), C.POINTER(C.c_int64)]),
     "bsar_server_poll": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(JobInfoC)]),
     "bsar_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(JobInfoC)]),
+    "bsar_server_wait_any": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(JobInfoC)]),
     "bsar_server_drain": (C.c_int, [C.c_void_p]),
     "bsar_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStatsC)]),
 }
@@ -464,6 +465,19 @@ class Server:
         if st != 0:
             raise BsarError(st, f"job {jid}: {self._err()}")
         return out, _info(ci)
+
+    def wait_any(self, timeout: float = -1.0):
+        """Next finished job: (id, image, info)."""
+        ci = JobInfoC()
+        st = self.L.bsar_server_wait_any(self.h, float(timeout), C.byref(ci))
+        if st == -7 and ci.id == 0:
+            raise TimeoutError(f"no job done after {timeout} s")
+        if ci.id == 0:
+            raise BsarError(st, f"server_wait_any: {self._err()}")
+        out, _ = self._jobs.pop(ci.id, (None, None))
+        if st != 0:
+            raise BsarError(st, f"job {ci.id}: {self._err()}")
+        return ci.id, out, _info(ci)
 
     def drain(self):
         st = self.L.bsar_server_drain(self.h)

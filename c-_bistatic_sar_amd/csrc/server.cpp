@@ -718,6 +718,32 @@ int bsar_server_wait(bsar_server* s, int64_t id, double timeout_s, bsar_job_info
     return status;
 }
 
+int bsar_server_wait_any(bsar_server* s, double timeout_s, bsar_job_info* info)
+{
+    if (!s || !info) return BSAR_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->jobs.empty()) { s->err = "no uncollected job"; return BSAR_ERR_INVALID; }
+    Job* found = nullptr;
+    auto any = [&] {
+        for (auto& kv : s->jobs)
+            if (kv.second->state == BSAR_JOB_DONE || kv.second->state == BSAR_JOB_FAILED) {
+                if (!found || kv.second->t_done < found->t_done) found = kv.second;
+            }
+        return found != nullptr;
+    };
+    if (timeout_s < 0) {
+        s->cv_done.wait(lk, any);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), any)) {
+        std::memset(info, 0, sizeof *info);
+        return BSAR_ERR_STATE;
+    }
+    fill_info(found, info);
+    const int status = found->status;
+    s->jobs.erase(found->id);
+    delete found;
+    return status;
+}
+
 int bsar_server_drain(bsar_server* s)
 {
     if (!s) return BSAR_ERR_INVALID;

@@ -265,6 +265,10 @@ BSAR_API int bsar_server_poll(bsar_server* s, int64_t id, bsar_job_info* info);
  * collects it (its id becomes unknown).  Returns the job's status, or
  * BSAR_ERR_STATE on timeout. */
 BSAR_API int bsar_server_wait(bsar_server* s, int64_t id, double timeout_s, bsar_job_info* info);
+/* Blocks until any uncollected job is done or failed, collects it and
+ * reports it in *info (info->id names it).  Returns that job's status,
+ * BSAR_ERR_STATE on timeout, BSAR_ERR_INVALID when no job is uncollected. */
+BSAR_API int bsar_server_wait_any(bsar_server* s, double timeout_s, bsar_job_info* info);
 /* Blocks until no job is queued or running. */
 BSAR_API int bsar_server_drain(bsar_server* s);
 BSAR_API int bsar_server_get_stats(bsar_server* s, bsar_server_stats* st);

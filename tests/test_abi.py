@@ -64,6 +64,7 @@ def test_server_without_device_fails_cleanly(bsar):
     L.bsar_server_destroy(None)                      # null is a no-op
     assert L.bsar_server_submit(None, None, None) == -1
     assert L.bsar_server_wait(None, 1, 0.0, None) == -1
+    assert L.bsar_server_wait_any(None, 0.0, None) == -1
     assert L.bsar_server_drain(None) == -1
     assert L.bsar_server_last_error(None) == b"null server"
 

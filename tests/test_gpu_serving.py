@@ -70,7 +70,14 @@ def _run(bsar, world, jobs, round_seconds, together):
         if together:
             ids = [srv.submit(world[k], *_grid(world["t"], g), pulses=K, precision=_prec(bsar, p))
                    for k, g, K, p in jobs]
-            out = [srv.wait(i, timeout=120) for i in ids]
+            got = {}
+            for _ in ids:                                  # collected in completion order
+                jid, img, info = srv.wait_any(timeout=120)
+                got[jid] = (img, info)
+            assert sorted(got) == sorted(ids)
+            out = [got[i] for i in ids]
+            with pytest.raises(bsar.BsarError):
+                srv.wait_any(timeout=0.1)                  # nothing left to collect
             st = srv.stats()
             assert st["completed"] == len(jobs) and st["failed"] == 0
             assert st["rounds"] >= 1
