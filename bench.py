@@ -6,6 +6,7 @@ on the same host.
 
     python bench.py [--steps K] [--warmup W] [--workload large|small|...]
     python bench.py --workload serve [--jobs 256] [--rate 40] [--no-fp64]
+    python bench.py --workload rough [--chunk 65536] [--check-solves 100000]
 
 One step = one full mixed-precision image of the workload with the
 range-compressed data already resident in HBM.  Prints one JSON line on
@@ -260,6 +261,100 @@ def run_serve(args, pkg, scene) -> dict:
     return out
 
 
+FLOP_PER_SOLVE = 709 * 46 + 64     # one refraction search: 709 candidates x 46 flop + set-up
+
+
+def run_rough(args, pkg, scene) -> dict:
+    """Refraction-dominated workload: every refraction solve of a 128^3 grid
+    buried 3-7 m under rough fBm terrain (167x100 DEM) for 1024 pulses, both
+    legs (2 V N = 4.29e9 solves) through the device-buffer slant-range entry
+    point, in voxel chunks; then the argmin of a seeded sample of >= 1e5
+    solves against the reference's own calculateRefractionPoint on the host
+    (identical point, delay difference <= 1e-12 s)."""
+    sc = scene.make_scene("rough")
+    V, N = sc.voxels, sc.n
+    nx, ny = len(sc.gx), len(sc.gy)
+    log(f"workload rough: {nx}x{ny}x{len(sc.gz)} voxels x {N} pulses, DEM {len(sc.dem[0])}x{len(sc.dem[1])}, "
+        f"n2 = {sc.n2:.4f}")
+    chunk = min(args.chunk, V)
+    rng = np.random.default_rng(1003 + 7)
+    per_chunk = max(1, -(-args.check_solves // (2 * N * (-(-V // chunk)))))
+    dev_s, legs_a, legs_b, got_a, got_b = 0.0, [], [], [], []
+    with pkg.Context(0) as ctx:
+        ds = ctx.dataset(pkg.Params(sc.fc, sc.c, sc.fs, sc.n1, sc.n2, sc.upsample), pkg.Dem(*sc.dem), sc.tx, sc.rx)
+        bp = pkg.DeviceBuffer(chunk * 3 * 8)
+        ba = pkg.DeviceBuffer(chunk * N * 3 * 8)
+        bb = pkg.DeviceBuffer(chunk * N * 3 * 8)
+        t_start = time.perf_counter()
+        for c0 in range(0, V, chunk):
+            m = min(chunk, V - c0)
+            v = np.arange(c0, c0 + m)
+            pts = np.stack([sc.gx[v % nx], sc.gy[(v // nx) % ny], sc.gz[v // (nx * ny)]], 1)
+            bp.write(pts)
+            rep = ds.slant_range_fermat_device(bp.ptr, m, ba.ptr, bb.ptr)
+            dev_s += rep["seconds"]
+            for vl in rng.choice(m, size=min(per_chunk, m), replace=False):
+                qa = ba.read((N, 3), offset=int(vl) * N * 24)
+                qb = bb.read((N, 3), offset=int(vl) * N * 24)
+                P = np.broadcast_to(pts[vl], (N, 3))
+                legs_a.append(np.concatenate([sc.tx, P], 1))
+                legs_b.append(np.concatenate([P, sc.rx], 1))
+                got_a.append(qa)
+                got_b.append(qb)
+            log(f"chunk {c0 // chunk + 1}/{-(-V // chunk)}: {m} voxels x {N} pulses x 2 legs in "
+                f"{rep['seconds']:.3f} s ({2 * m * N / rep['seconds']:.3e} solves/s)")
+        wall = time.perf_counter() - t_start
+        for b in (bp, ba, bb):
+            b.close()
+        ds.close()
+    solves = 2.0 * V * N
+    value = solves / dev_s
+    la, lb = np.concatenate(legs_a), np.concatenate(legs_b)
+    ga, gb = np.concatenate(got_a), np.concatenate(got_b)
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    t0 = time.time()
+    ra = O.ref_refraction(sc.dem, la, sc.n1, sc.n2, threads=threads)
+    kind = "reference"
+    if ra is None:
+        kind = "port"
+        ra = O.refraction_batch(sc.dem, la, sc.n1, sc.n2, threads=threads)
+        rb = O.refraction_batch(sc.dem, lb, sc.n2, sc.n1, threads=threads)
+    else:
+        rb = O.ref_refraction(sc.dem, lb, sc.n2, sc.n1, threads=threads)
+    cpu_s = time.time() - t0
+    n_check = len(la) + len(lb)
+
+    def delay(legs, q, na, nb):
+        d1 = np.sqrt(((legs[:, 0:3] - q) ** 2).sum(1))
+        d2 = np.sqrt(((q - legs[:, 3:6]) ** 2).sum(1))
+        return (na * d1 + nb * d2) / sc.c
+
+    same = int((ga == ra).all(1).sum() + (gb == rb).all(1).sum())
+    dtau = max(float(np.abs(delay(la, ga, sc.n1, sc.n2) - delay(la, ra, sc.n1, sc.n2)).max()),
+               float(np.abs(delay(lb, gb, sc.n2, sc.n1) - delay(lb, rb, sc.n2, sc.n1)).max()))
+    log(f"argmin check: {same}/{n_check} identical, max |dtau| = {dtau:.3e} s ({kind} CPU, {cpu_s:.1f} s)")
+    ach = value * FLOP_PER_SOLVE
+    return {"metric": "refraction solves/s (Fermat search over the DEM, FP64), both legs of every voxel-pulse",
+            "value": round(value, 1), "unit": "solve/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": round(dev_s * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded fBm DEM, spiral antennas)",
+            "config": {"workload": f"rough: {nx}x{ny}x{len(sc.gz)} grid 3-7 m under a {len(sc.dem[0])}x"
+                                   f"{len(sc.dem[1])} fBm DEM (H 0.7, 40 m RMS), {N} pulses, eps_r "
+                                   f"{sc.n2 ** 2:.3f}", "voxels": V, "pulses": N, "solves": solves,
+                       "chunk_voxels": chunk},
+            "wall_s": round(wall, 2),
+            "roofline": {"bound": "valu", "achieved": round(ach / 1e12, 3), "peak": P64 / 1e12,
+                         "unit": "TFLOP/s", "frac": round(ach / P64, 4), "traffic": None,
+                         "kernel": "k_slant_range", "flop_per_solve": FLOP_PER_SOLVE},
+            "argmin_check": {"solves": n_check, "identical": same, "max_abs_dtau_s": dtau, "cpu": kind},
+            "cpu_baseline": {"value": round(n_check / cpu_s, 1), "unit": "solve/s", "cores": threads, "kind": kind,
+                             "sample": f"{n_check} solves (the argmin-check sample: {len(legs_a)} voxels x "
+                                       f"{N} pulses x 2 legs), calculateRefractionPoint over {threads} threads"},
+            "vs_cpu": round(value / (n_check / cpu_s), 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,6 +366,8 @@ def main() -> int:
     ap.add_argument("--jobs", type=int, default=256, help="serve: jobs in the stream")
     ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate, jobs/s")
     ap.add_argument("--round-ms", type=float, default=40.0, help="serve: target device time per round")
+    ap.add_argument("--chunk", type=int, default=65536, help="rough: voxels per refraction launch")
+    ap.add_argument("--check-solves", type=int, default=100000, help="rough: solves checked against the CPU")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -281,6 +378,9 @@ def main() -> int:
     scene = load("bsar_amd.scene", PKG_DIR / "scene.py")
     if args.workload == "serve":
         print(json.dumps(run_serve(args, pkg, scene)), flush=True)
+        return 0
+    if args.workload == "rough":
+        print(json.dumps(run_rough(args, pkg, scene)), flush=True)
         return 0
     sc = scene.make_scene(args.workload)
     V, N = sc.voxels, sc.n

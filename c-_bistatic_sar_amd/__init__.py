@@ -89,6 +89,8 @@ SIGNATURES = {
     "bsar_dem_interp": (C.c_int, [C.c_void_p, C.POINTER(DemC), _dp, C.c_int64, _dp]),
     "bsar_refraction_points": (C.c_int, [C.c_void_p, C.POINTER(DemC), _dp, C.c_int64, C.c_double, C.c_double, _dp]),
     "bsar_slant_range_fermat": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int64, _dp, _dp]),
+    "bsar_slant_range_fermat_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                                 C.c_void_p, C.POINTER(ReportC)]),
     "bsar_upsample": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp]),
     "bsar_range_compress": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp]),
     "bsar_chirp": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
@@ -322,6 +324,15 @@ class Dataset:
                       "slant_range_fermat")
         return ida, volta
 
+    def slant_range_fermat_device(self, d_pts: int, V: int, d_ida: int, d_volta: int) -> dict:
+        """Device-pointer form (caller-owned buffers: pts [V][3], ida/volta [V][N][3]);
+        returns the run report."""
+        rep = ReportC()
+        self.ctx._chk(self.L.bsar_slant_range_fermat_device(self.ctx.h, self.h, C.c_void_p(d_pts), V,
+                                                            C.c_void_p(d_ida), C.c_void_p(d_volta), C.byref(rep)),
+                      "slant_range_fermat_device")
+        return {f: getattr(rep, f) for f, _ in ReportC._fields_}
+
     def backproject(self, gx, gy, gz, precision: int = FP64):
         """Image [nz][ny][nx] complex (x fastest) and the run report (dict)."""
         gx, gy, gz = _f64(gx), _f64(gy), _f64(gz)
@@ -382,6 +393,77 @@ def write_matrix2d(path, data):
 
 def flops_per_voxel_pulse(n: int) -> float:
     return float(lib().bsar_flops_per_voxel_pulse(n))
+
+
+_hip = None
+
+
+def _hiprt() -> C.CDLL:
+    """The HIP runtime libbsar.so is linked against (device buffers for the
+    *_device entry points)."""
+    global _hip
+    if _hip is None:
+        lib()
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                H = C.CDLL(name)
+                break
+            except OSError:
+                H = None
+        if H is None:
+            raise ImportError("libamdhip64.so not found")
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip = H
+    return _hip
+
+
+class DeviceBuffer:
+    """A device allocation (hipMalloc) owned by Python."""
+
+    def __init__(self, nbytes: int):
+        self.H = _hiprt()
+        p = C.c_void_p()
+        st = self.H.hipMalloc(C.byref(p), max(int(nbytes), 1))
+        if st != 0:
+            raise BsarError(-3, f"hipMalloc({nbytes}) failed with HIP error {st}")
+        self.ptr, self.nbytes = p.value, int(nbytes)
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.write(a)
+        return b
+
+    def write(self, a: np.ndarray, offset: int = 0):
+        a = np.ascontiguousarray(a)
+        if offset + a.nbytes > self.nbytes:
+            raise ValueError("write past the end of the buffer")
+        st = self.H.hipMemcpy(C.c_void_p(self.ptr + offset), a.ctypes.data_as(C.c_void_p), a.nbytes, 1)
+        if st != 0:
+            raise BsarError(-2, f"hipMemcpy H2D failed with HIP error {st}")
+
+    def read(self, shape, dtype=np.float64, offset: int = 0) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        if offset + out.nbytes > self.nbytes:
+            raise ValueError("read past the end of the buffer")
+        st = self.H.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + offset), out.nbytes, 2)
+        if st != 0:
+            raise BsarError(-2, f"hipMemcpy D2H failed with HIP error {st}")
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.H.hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}

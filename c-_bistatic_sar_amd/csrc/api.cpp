@@ -370,6 +370,31 @@ int bsar_slant_range_fermat(bsar_ctx* c, const bsar_dataset* ds, const double* p
     return BSAR_OK;
 }
 
+int bsar_slant_range_fermat_device(bsar_ctx* c, const bsar_dataset* ds, const double* d_pts, int64_t V,
+                                   double* d_ida, double* d_volta, bsar_report* rep)
+{
+    if (!c || !ds) return BSAR_ERR_INVALID;
+    if (V < 0 || (V && (!d_pts || !d_ida || !d_volta))) return set_error(c, BSAR_ERR_INVALID, "bad device buffer");
+    if (ds->device != c->device) return set_error(c, BSAR_ERR_INVALID, "dataset lives on another device");
+    BSAR_HIP(c, hipSetDevice(c->device));
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    BSAR_HIP(c, hipEventRecord(c->ev0, c->stream));
+    if (V > 0 && ds->n > 0)
+        BSAR_HIP(c, launch_slant_range(ds->dem, ds->traj, d_pts, V, ds->p.n1, ds->p.n2, d_ida, d_volta, c->stream));
+    BSAR_HIP(c, hipEventRecord(c->ev1, c->stream));
+    BSAR_HIP(c, hipEventSynchronize(c->ev1));
+    if (rep) {
+        float ms = 0.f;
+        BSAR_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        rep->seconds = ms * 1e-3;
+        rep->voxel_pulses = (double)V * ds->n;
+        rep->voxels = V;
+        rep->pulses = ds->n;
+        rep->launches = (V > 0 && ds->n > 0) ? 1 : 0;
+    }
+    return BSAR_OK;
+}
+
 int bsar_upsample(bsar_ctx* c, const double* in, int S, int N, int factor, double* out)
 {
     if (!c) return BSAR_ERR_INVALID;

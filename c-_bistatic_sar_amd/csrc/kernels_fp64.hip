@@ -47,7 +47,10 @@ __global__ __launch_bounds__(256, 2) void k_refraction(DemDev dem, const double*
 }
 
 // Thread per (point v, pulse i): both legs of one calculaSlantRangeFermat
-// iteration (raw_data_processing.cpp:166-181).
+// iteration (raw_data_processing.cpp:166-181).  Points run fastest across
+// threads, so a wavefront searches around one antenna position for
+// neighbouring points (coherent DEM accesses and screening outcomes); the
+// outputs are [V][N][3] as calculaSlantRangeFermat returns them per target.
 __global__ __launch_bounds__(256, 2) void k_slant_range(DemDev dem, Traj tr, const double* __restrict__ pts,
                                                      int64_t V, double n1, double n2,
                                                      double* __restrict__ ida, double* __restrict__ volta)
@@ -60,14 +63,15 @@ __global__ __launch_bounds__(256, 2) void k_slant_range(DemDev dem, Traj tr, con
     }
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= V * tr.n) return;
-    int64_t v = k / tr.n;
-    int i = (int)(k - v * tr.n);
+    const int i = (int)(k / V);
+    const int64_t v = k - (int64_t)i * V;
+    const int64_t o = 3 * (v * tr.n + i);
     const double Px = pts[3 * v], Py = pts[3 * v + 1], Pz = pts[3 * v + 2];
     double qx, qy, qz;
     fermat_solve(d, tr.tx[i], tr.ty[i], tr.tz[i], Px, Py, Pz, n1, n2, qx, qy, qz);
-    ida[3 * k] = qx; ida[3 * k + 1] = qy; ida[3 * k + 2] = qz;
+    ida[o] = qx; ida[o + 1] = qy; ida[o + 2] = qz;
     fermat_solve(d, Px, Py, Pz, tr.rx[i], tr.ry[i], tr.rz[i], n2, n1, qx, qy, qz);
-    volta[3 * k] = qx; volta[3 * k + 1] = qy; volta[3 * k + 2] = qz;
+    volta[o] = qx; volta[o + 1] = qy; volta[o + 2] = qz;
 }
 
 // splineInterpolation, interpolation.cpp:4-51: one thread per output sample.

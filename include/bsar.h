@@ -115,6 +115,12 @@ BSAR_API int bsar_refraction_points(bsar_ctx* ctx, const bsar_dem* dem, const do
 BSAR_API int bsar_slant_range_fermat(bsar_ctx* ctx, const bsar_dataset* ds, const double* pts, int64_t V,
                             double* ida, double* volta);
 
+/* Same with caller-owned device buffers (d_pts [V][3], d_ida / d_volta
+ * [V][N][3]) for callers that keep the refraction points on the GPU; rep
+ * (optional) receives the kernel time (voxel_pulses = V*N, two solves each). */
+BSAR_API int bsar_slant_range_fermat_device(bsar_ctx* ctx, const bsar_dataset* ds, const double* d_pts, int64_t V,
+                                   double* d_ida, double* d_volta, bsar_report* rep);
+
 /* splineInterpolation, interpolation.cpp:4-51 (declared interpolation.h:8),
  * over N pulses at once as proc_test_script.cpp:326-350 drives it:
  * in [N][S] complex (range fastest), out [N][S*factor] complex.  S >= 2. */

@@ -384,6 +384,17 @@ void or_simulate(const or_sim_params* p, const or_dem* dem,
     free(aux); free(chirp); free(refsig); free(col); free(res); free(raw);
 }
 
+void or_refraction_batch(const double* legs, size_t m, const or_dem* dem, double n1, double n2,
+                         double* out, int threads)
+{
+    if (threads > 0) omp_set_num_threads(threads);
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (long long k = 0; k < (long long)m; k++) {
+        const double* s = legs + 6 * (size_t)k;
+        or_refraction_point(s[0], s[1], s[2], s[3], s[4], s[5], dem, n1, n2, out + 3 * (size_t)k);
+    }
+}
+
 /* proc_test_script.cpp:326-350: pulse p of rootData is flat[r + p*S] */
 void or_upsample(const double* root, size_t S, size_t N, int factor, double* out)
 {

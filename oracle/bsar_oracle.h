@@ -54,6 +54,11 @@ void or_refraction_point(double x1, double y1, double z1,
                          const or_dem* dem, double n1, double n2,
                          double out[3]);
 
+/* or_refraction_point over m independent legs legs[k] = (x1,y1,z1,x2,y2,z2),
+ * OpenMP over legs (threads <= 0: the runtime default). */
+void or_refraction_batch(const double* legs, size_t m, const or_dem* dem, double n1, double n2,
+                         double* out, int threads);
+
 /* calculateReflectionPoint, raw_data_processing.cpp:575-612 (dead for the
  * image; run only so CPU timing matches the reference's work). */
 void or_reflection_point(double txX, double txY, double txZ,

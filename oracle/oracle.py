@@ -67,6 +67,7 @@ def lib() -> C.CDLL:
         L.or_simulate.argtypes = [C.POINTER(SimParams), C.POINTER(DemO), _dp, _dp, C.c_int, _dp, _dp, C.c_int,
                                   _dp, _dp, _dp]
         L.or_upsample.argtypes = [_dp, C.c_size_t, C.c_size_t, C.c_int, _dp]
+        L.or_refraction_batch.argtypes = [_dp, C.c_size_t, C.POINTER(DemO), C.c_double, C.c_double, _dp, C.c_int]
         L.or_backproject.argtypes = [C.POINTER(BpParams), C.POINTER(DemO), _dp, _dp, C.c_int, _dp, C.c_size_t,
                                      C.c_size_t, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp,
                                      C.POINTER(C.c_int64), C.c_int64, C.c_int, C.c_int]
@@ -111,6 +112,39 @@ def refraction_points(dem, legs, n1, n2):
     for k, s in enumerate(legs):
         L.or_refraction_point(*s, C.byref(d.s), n1, n2, _p(q))
         out[k] = q
+    return out
+
+
+def refraction_batch(dem, legs, n1, n2, threads=0):
+    """refraction_points over many legs in C with OpenMP."""
+    d = _DemHold(*dem)
+    legs = _f(legs).reshape(-1, 6)
+    out = np.empty((len(legs), 3))
+    lib().or_refraction_batch(_p(legs), len(legs), C.byref(d.s), n1, n2, _p(out), int(threads))
+    return out
+
+
+def ref_refraction(dem, legs, n1, n2, threads=1):
+    """The reference's calculateRefractionPoint (oracle/_ref/libref.so) over
+    legs, split across `threads` host threads (the function is reentrant);
+    None when the reference library has not been built."""
+    L = ref_lib()
+    if L is None:
+        return None
+    import concurrent.futures as cf
+    x, y, z = _f(dem[0]), _f(dem[1]), _f(dem[2])
+    zr, zc = z.shape if z.ndim == 2 else (0, 0)
+    legs = _f(legs).reshape(-1, 6)
+    out = np.empty((len(legs), 3))
+    bounds = np.linspace(0, len(legs), max(int(threads), 1) + 1).astype(int)
+
+    def run(a, b):
+        if b > a:
+            L.ref_refraction(_p(x), len(x), _p(y), len(y), _p(z), zr, zc, _p(legs[a:b]), b - a, n1, n2,
+                             out[a:b].ctypes.data_as(_dp))
+
+    with cf.ThreadPoolExecutor(max_workers=len(bounds) - 1) as ex:
+        list(ex.map(lambda ab: run(*ab), zip(bounds[:-1], bounds[1:])))
     return out
 
 

@@ -65,6 +65,34 @@ def test_rough_dem_refraction_vs_oracle(bsar, ctx, oracle, scene_mod):
     assert bits_equal(gpu, ref)
 
 
+def test_rough_scene_slant_range_device_vs_oracle(bsar, ctx, oracle, scene_mod):
+    """The rough workload's geometry (grid 3-7 m under 167 x 100 fBm terrain,
+    antennas 12 m above it) on a sub-grid: the device-buffer entry point, the
+    host entry point and the oracle give bit-identical refraction points for
+    every (voxel, pulse, leg)."""
+    sc = scene_mod.make_scene("rough", n_pulses=40, grid=(11, 9, 7))
+    V, N = sc.voxels, sc.n
+    nx, ny = len(sc.gx), len(sc.gy)
+    v = np.arange(V)
+    pts = np.stack([sc.gx[v % nx], sc.gy[(v // nx) % ny], sc.gz[v // (nx * ny)]], 1)
+    ds = ctx.dataset(bsar.Params(sc.fc, sc.c, sc.fs, sc.n1, sc.n2), bsar.Dem(*sc.dem), sc.tx, sc.rx)
+    bp = bsar.DeviceBuffer.from_array(pts)
+    ba, bb = bsar.DeviceBuffer(V * N * 24), bsar.DeviceBuffer(V * N * 24)
+    rep = ds.slant_range_fermat_device(bp.ptr, V, ba.ptr, bb.ptr)
+    ida, volta = ba.read((V, N, 3)), bb.read((V, N, 3))
+    for b in (bp, ba, bb):
+        b.close()
+    assert rep["voxel_pulses"] == V * N and rep["launches"] == 1 and rep["seconds"] > 0
+    hida, hvolta = ds.slant_range_fermat(pts)
+    ds.close()
+    assert bits_equal(ida, hida) and bits_equal(volta, hvolta)
+    P = np.repeat(pts, N, axis=0)
+    legs_a = np.concatenate([np.tile(sc.tx, (V, 1)), P], 1)
+    legs_b = np.concatenate([P, np.tile(sc.rx, (V, 1))], 1)
+    assert bits_equal(ida.reshape(-1, 3), oracle.refraction_batch(sc.dem, legs_a, sc.n1, sc.n2, threads=8))
+    assert bits_equal(volta.reshape(-1, 3), oracle.refraction_batch(sc.dem, legs_b, sc.n2, sc.n1, threads=8))
+
+
 def test_slant_range_bit_exact(bsar, ctx, golden):
     g = golden
     p = bsar.Params(1e9, 3e8, 800e6, 1.0, 2.449489742783178)
