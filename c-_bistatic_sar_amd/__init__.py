@@ -93,11 +93,15 @@ SIGNATURES = {
                                                  C.c_void_p, C.POINTER(ReportC)]),
     "bsar_upsample": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp]),
     "bsar_range_compress": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp]),
+    "bsar_range_compress_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_int,
+                                             C.POINTER(ReportC)]),
     "bsar_chirp": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
     "bsar_dataset_create": (C.c_int, [C.c_void_p, C.POINTER(ParamsC), C.POINTER(DemC), _dp, _dp, C.c_int,
                                       C.POINTER(C.c_void_p)]),
     "bsar_dataset_destroy": (None, [C.c_void_p]),
     "bsar_dataset_load_rootdata": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int, C.c_int]),
+    "bsar_dataset_load_rootdata_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                                    C.c_int]),
     "bsar_dataset_load_upsampled": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int, C.c_int]),
     "bsar_dataset_device_bytes": (C.c_int64, [C.c_void_p]),
     "bsar_backproject": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp,
@@ -264,6 +268,21 @@ class Context:
                                              int(conj), precision, _ptr(out.view(np.float64))), "range_compress")
         return out
 
+    def range_compress_device(self, buf: "DeviceBuffer", S: int, N: int, ref, conj: bool = True,
+                              precision: int = FP64) -> dict:
+        """In place on a device buffer of [N][S] complex128 (FP64) / complex64
+        (MIXED); returns the run report (device time of the filter)."""
+        esz = 16 if precision == FP64 else 8
+        if buf.nbytes < S * N * esz:
+            raise ValueError("device buffer too small")
+        r = _c128(ref).reshape(-1)
+        if len(r) != S:
+            raise ValueError("reference length must equal the column length")
+        rep = ReportC()
+        self._chk(self.L.bsar_range_compress_device(self.h, C.c_void_p(buf.ptr), S, N, _ptr(r.view(np.float64)),
+                                                    int(conj), precision, C.byref(rep)), "range_compress_device")
+        return {f: getattr(rep, f) for f, _ in ReportC._fields_}
+
     def dataset(self, params: Params, dem: Dem, tx, rx) -> "Dataset":
         return Dataset(self, params, dem, tx, rx)
 
@@ -306,6 +325,14 @@ class Dataset:
         nd, S = r.shape
         self.ctx._chk(self.L.bsar_dataset_load_rootdata(self.ctx.h, self.h, _ptr(r.view(np.float64)), S, nd),
                       "load_rootdata")
+
+    def load_rootdata_device(self, buf: "DeviceBuffer", S: int, n_data: int, precision: int = FP64):
+        """rootData from a device buffer ([n_data][S] complex128, or complex64 with MIXED)."""
+        esz = 16 if precision == FP64 else 8
+        if buf.nbytes < S * n_data * esz:
+            raise ValueError("device buffer too small")
+        self.ctx._chk(self.L.bsar_dataset_load_rootdata_device(self.ctx.h, self.h, C.c_void_p(buf.ptr), S, n_data,
+                                                               precision), "load_rootdata_device")
 
     def load_upsampled(self, rc):
         r = _c128(rc)

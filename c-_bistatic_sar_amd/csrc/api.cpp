@@ -1,6 +1,7 @@
 // api.cpp -- C ABI of the imaging hot path (declarations and reference
 // citations in include/bsar.h).
 #include "api_internal.h"
+#include "bsar_fft.h"
 
 #include <algorithm>
 #include <cmath>
@@ -533,6 +534,32 @@ int bsar_dataset_load_rootdata(bsar_ctx* c, bsar_dataset* ds, const double* root
     BSAR_HIP(c, hipMemcpyAsync(bi.p, root, nin * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     BSAR_HIP(c, hipMalloc((void**)&ds->rc, nin * ds->p.upsample * sizeof(double2)));
     BSAR_HIP(c, launch_spline((const double2*)bi.p, S, n_data, ds->p.upsample, ds->rc, c->stream));
+    ds->ndata = n_data;
+    ds->S4 = (int)S4;
+    return finish_rcomp(c, ds);
+}
+
+int bsar_dataset_load_rootdata_device(bsar_ctx* c, bsar_dataset* ds, const void* d_root, int S, int n_data,
+                                      int precision)
+{
+    if (!c || !ds) return BSAR_ERR_INVALID;
+    if (S < 2 || n_data < 1 || !d_root) return set_error(c, BSAR_ERR_INVALID, "rootData needs S >= 2, n_data >= 1");
+    if (precision != BSAR_FP64 && precision != BSAR_MIXED) return set_error(c, BSAR_ERR_INVALID, "bad precision");
+    if (ds->device != c->device) return set_error(c, BSAR_ERR_INVALID, "dataset lives on another device");
+    const int64_t S4 = (int64_t)S * ds->p.upsample;
+    if (S4 > std::numeric_limits<int>::max() / 2) return set_error(c, BSAR_ERR_INVALID, "S * upsample too large");
+    BSAR_HIP(c, hipSetDevice(c->device));
+    if (ds->rc) drop_rcomp(ds);
+    const size_t nin = (size_t)S * n_data;
+    DevBuf wide;
+    const double2* src = (const double2*)d_root;
+    if (precision == BSAR_MIXED) {
+        BSAR_HIP(c, hipMalloc(&wide.p, nin * sizeof(double2)));
+        BSAR_HIP(c, to_f64((const float2*)d_root, (double2*)wide.p, (int64_t)nin, c->stream));
+        src = (const double2*)wide.p;
+    }
+    BSAR_HIP(c, hipMalloc((void**)&ds->rc, nin * ds->p.upsample * sizeof(double2)));
+    BSAR_HIP(c, launch_spline(src, S, n_data, ds->p.upsample, ds->rc, c->stream));
     ds->ndata = n_data;
     ds->S4 = (int)S4;
     return finish_rcomp(c, ds);

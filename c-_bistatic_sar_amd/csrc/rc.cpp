@@ -85,3 +85,47 @@ extern "C" int bsar_range_compress(bsar_ctx* c, const double* sig, int S, int N,
     }
     return BSAR_OK;
 }
+
+extern "C" int bsar_range_compress_device(bsar_ctx* c, void* d_sig, int S, int N, const double* ref, int conj,
+                                          int precision, bsar_report* rep)
+{
+    if (!c) return BSAR_ERR_INVALID;
+    if (S < 2 || (S & (S - 1)) || N < 0 || (N && !d_sig) || !ref)
+        return set_error(c, BSAR_ERR_INVALID, "range_compress_device needs a power-of-two S >= 2 and buffers");
+    if (precision != BSAR_FP64 && precision != BSAR_MIXED) return set_error(c, BSAR_ERR_INVALID, "bad precision");
+    if (S > (1 << 26)) return set_error(c, BSAR_ERR_INVALID, "column too long");
+    BSAR_CHECK_HIP(c, hipSetDevice(c->device));
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    if (N == 0) return BSAR_OK;
+    const size_t P = (size_t)S, n = P * (size_t)N;
+    // filter spectrum H = FFT(ref) (conjugated for correlation), FP64
+    Buf dref, dtmp1, h32, tmp;
+    BSAR_CHECK_HIP(c, hipMalloc(&dref.p, P * sizeof(double2)));
+    BSAR_CHECK_HIP(c, hipMalloc(&dtmp1.p, P * sizeof(double2)));
+    BSAR_CHECK_HIP(c, hipMemcpyAsync(dref.p, ref, P * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    double2* H = nullptr;
+    BSAR_CHECK_HIP(c, fft_forward((double2*)dref.p, (double2*)dtmp1.p, (int)P, 1, c->stream, &H));
+    if (conj) BSAR_CHECK_HIP(c, conj_inplace(H, (int64_t)P, c->stream));
+    const size_t esz = precision == BSAR_FP64 ? sizeof(double2) : sizeof(float2);
+    if (P * esz > kFftLdsMax) BSAR_CHECK_HIP(c, hipMalloc(&tmp.p, n * esz));
+    if (precision == BSAR_MIXED) {
+        BSAR_CHECK_HIP(c, hipMalloc(&h32.p, P * sizeof(float2)));
+        BSAR_CHECK_HIP(c, to_f32(H, (float2*)h32.p, (int64_t)P, c->stream));
+    }
+    BSAR_CHECK_HIP(c, hipEventRecord(c->ev0, c->stream));
+    if (precision == BSAR_FP64)
+        BSAR_CHECK_HIP(c, fft_filter((double2*)d_sig, (double2*)tmp.p, H, (int)P, N, c->stream));
+    else
+        BSAR_CHECK_HIP(c, fft_filter((float2*)d_sig, (float2*)tmp.p, (const float2*)h32.p, (int)P, N, c->stream));
+    BSAR_CHECK_HIP(c, hipEventRecord(c->ev1, c->stream));
+    BSAR_CHECK_HIP(c, hipEventSynchronize(c->ev1));
+    if (rep) {
+        float ms = 0.f;
+        BSAR_CHECK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        rep->seconds = ms * 1e-3;
+        rep->pulses = N;
+        rep->voxels = 0;
+        rep->launches = 1;
+    }
+    return BSAR_OK;
+}

@@ -136,6 +136,13 @@ BSAR_API int bsar_upsample(bsar_ctx* ctx, const double* in, int S, int N, int fa
 BSAR_API int bsar_range_compress(bsar_ctx* ctx, const double* sig, int S, int N, const double* ref,
                         int conj, int precision, double* out);
 
+/* Same filter on a caller-owned device buffer, in place: d_sig is [N][S]
+ * complex128 (BSAR_FP64) or complex64 (BSAR_MIXED), S a power of two; ref is
+ * the host reference (complex128, length S).  rep (optional) receives the
+ * device time of the batched FFT -> multiply -> IFFT. */
+BSAR_API int bsar_range_compress_device(bsar_ctx* ctx, void* d_sig, int S, int N, const double* ref, int conj,
+                               int precision, bsar_report* rep);
+
 /* createCHIRP, raw_data_processing.cpp:257-290: writes L = int(T*fs)+1
  * samples when out != NULL; returns L (> 0) or a negative status. */
 BSAR_API int bsar_chirp(double f0, double f1, double T, double fs, double* out);
@@ -153,6 +160,12 @@ BSAR_API void bsar_dataset_destroy(bsar_dataset* ds);
  * upsampled by params.upsample on the GPU (proc_test_script.cpp:326-350);
  * the upsampled block stays resident. */
 BSAR_API int bsar_dataset_load_rootdata(bsar_ctx* ctx, bsar_dataset* ds, const double* root, int S, int n_data);
+
+/* Same, from a caller-owned device buffer: d_root is [n_data][S] complex128
+ * (BSAR_FP64) or complex64 (BSAR_MIXED, widened to FP64 before the spline),
+ * e.g. the output of bsar_range_compress_device. */
+BSAR_API int bsar_dataset_load_rootdata_device(bsar_ctx* ctx, bsar_dataset* ds, const void* d_root, int S, int n_data,
+                                      int precision);
 
 /* Same, when the samples are already upsampled ([n_data][S4] complex). */
 BSAR_API int bsar_dataset_load_upsampled(bsar_ctx* ctx, bsar_dataset* ds, const double* rcomp, int S4, int n_data);

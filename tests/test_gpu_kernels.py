@@ -167,3 +167,49 @@ def test_pulse_compression_of_reference_simulation(bsar, ctx, golden):
     ref[S - L // 2:] = chirp[: L // 2]
     gpu = ctx.range_compress(raw, ref, conj=True)
     assert np.abs(gpu - root).max() <= 1e-12 * np.abs(root).max()
+
+
+@pytest.mark.parametrize("S", [4096, 16384])
+def test_range_compress_device_matches_host(bsar, ctx, S):
+    """The device-buffer entry point runs the host entry point's kernels: same
+    bits in FP64 (complex128 in place) and FP32 (complex64 in place)."""
+    rng = np.random.default_rng(S + 3)
+    N = 5
+    sig = rng.normal(size=(N, S)) + 1j * rng.normal(size=(N, S))
+    ref = rng.normal(size=S) + 1j * rng.normal(size=S)
+    b = bsar.DeviceBuffer.from_array(sig)
+    rep = ctx.range_compress_device(b, S, N, ref, conj=True, precision=bsar.FP64)
+    assert bits_equal(b.read((N, S), np.complex128).view(np.float64),
+                      ctx.range_compress(sig, ref, conj=True).view(np.float64))
+    assert rep["seconds"] > 0 and rep["pulses"] == N
+    b.close()
+    s32 = sig.astype(np.complex64)
+    b = bsar.DeviceBuffer.from_array(s32)
+    ctx.range_compress_device(b, S, N, ref, conj=False, precision=bsar.MIXED)
+    host = ctx.range_compress(s32.astype(np.complex128), ref, conj=False, precision=bsar.MIXED)
+    assert bits_equal(b.read((N, S), np.complex64).view(np.float32), host.astype(np.complex64).view(np.float32))
+    b.close()
+
+
+def test_load_rootdata_device_matches_host(bsar, ctx):
+    from _scenes import Tiny
+    t = Tiny(n=12, S=256, grid=(5, 4, 3))
+    root = t.random_root()
+    a = ctx.dataset(t.params(bsar), bsar.Dem(*t.dem), t.tx, t.rx)
+    b = ctx.dataset(t.params(bsar), bsar.Dem(*t.dem), t.tx, t.rx)
+    a.load_rootdata(root)
+    buf = bsar.DeviceBuffer.from_array(root)
+    b.load_rootdata_device(buf, t.S, len(root), bsar.FP64)
+    buf.close()
+    for p in (bsar.FP64, bsar.MIXED):
+        assert bits_equal(a.backproject(t.gx, t.gy, t.gz, p)[0].view(np.float64),
+                          b.backproject(t.gx, t.gy, t.gz, p)[0].view(np.float64))
+    r32 = root.astype(np.complex64)
+    a.load_rootdata(r32.astype(np.complex128))
+    buf = bsar.DeviceBuffer.from_array(r32)
+    b.load_rootdata_device(buf, t.S, len(root), bsar.MIXED)
+    buf.close()
+    assert bits_equal(a.backproject(t.gx, t.gy, t.gz, bsar.MIXED)[0].view(np.float64),
+                      b.backproject(t.gx, t.gy, t.gz, bsar.MIXED)[0].view(np.float64))
+    a.close()
+    b.close()
