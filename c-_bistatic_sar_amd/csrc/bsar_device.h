@@ -230,7 +230,6 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         pyv[j] = py;
     }
 
-    double runS = __builtin_inf();
 #pragma unroll 1
     for (int k = 0; k < G; ++k) {
         const int i = center_out(k, C);
@@ -243,44 +242,48 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
                                 n2 * sqrt_est(__builtin_fma(a2, a2, L.g2sq))) * kDefl;
             if (lbr > M) continue;
         }
+        // columns whose lower bound can still reach M (bit jj <-> column
+        // center_out(jj)); the probe (i, j) = (C, C) is bit 0 of row C
+        unsigned colok = 0u;
+#pragma unroll
+        for (int jj = 0; jj < G; ++jj)
+            colok |= (!((double)lbc[center_out(jj, C)] > M)) ? (1u << jj) : 0u;
+        if (i == C) colok &= ~1u;
+        if (!colok) continue;
         const double px = minX + i * spanX / (G - 1);
         int i0 = 0, di = 0;
         double wx = 0.0;
         if (!empty) axis_lookup(d.X, d.nx, d.x0, d.ihx, xsorted, px, i0, di, wx);
         const double dx1 = px - L.x1, dx2 = L.x2 - px;
         const double dxa = dx1 * dx1, dxb = dx2 * dx2;
-        // screening pass: bit jj set when candidate (i, center_out(jj)) may
-        // still be the minimum
-        unsigned pass = 0u;
+        // Screening pass, branch-free: every candidate of the row is
+        // estimated (loads unconditional: Z rows are padded, and an empty
+        // DEM reads its zero row), so the LDS loads of a row issue together.
+        // Bit jj of pass: candidate (i, center_out(jj)) may still be the minimum.
+        unsigned pass = colok;
+        if (L.screen) {
+            pass = 0u;
+            const double* zr = d.Z + i0;
 #pragma unroll
-        for (int jj = 0; jj < G; ++jj) {
-            const int j = center_out(jj, C);
-            if ((double)lbc[j] > M) continue;
-            if (i == C && j == C) continue;          // the probe
-            double ze = 0.0;
-            if (!empty) {
+            for (int jj = 0; jj < G; ++jj) {
+                const int j = center_out(jj, C);
                 const int r0o = jr[j] >> 1;
                 const int r1o = r0o + ((jr[j] & 1) ? d.zs : 0);
-                const double* r0 = d.Z + r0o + i0;
-                const double* r1 = d.Z + r1o + i0;
-                const double z00 = r0[0], z10 = r1[0];
-                const double z01 = di ? r0[1] : z00;
-                const double z11 = di ? r1[1] : z10;
+                const double z00 = zr[r0o], z01 = zr[r0o + di];
+                const double z10 = zr[r1o], z11 = zr[r1o + di];
                 const double za = __builtin_fma(z01 - z00, wx, z00);
                 const double zb = __builtin_fma(z11 - z10, wx, z10);
-                ze = __builtin_fma(zb - za, wy[j], za);
+                const double ze = __builtin_fma(zb - za, wy[j], za);
+                const double dy1 = pyv[j] - L.y1, dy2 = L.y2 - pyv[j];
+                const double e1 = ze - L.z1, e2 = L.z2 - ze;
+                const double A = __builtin_fma(e1, e1, __builtin_fma(dy1, dy1, dxa));
+                const double B = __builtin_fma(e2, e2, __builtin_fma(dy2, dy2, dxb));
+                const double os = __builtin_fma(n2, sqrt_est(B), n1 * sqrt_est(A));
+                const bool ok = (colok >> jj) & 1u;
+                // M = min(M, (smallest estimate so far + D) (1 + 3 kRho)); NaN estimates leave it
+                M = fmin(M, ((ok ? os : __builtin_inf()) + L.D) * (1.0 + 3.0 * kRho));
+                pass |= (ok && !(os > M * (1.0 + 2.0 * kRho) + L.D)) ? (1u << jj) : 0u;
             }
-            const double dy1 = pyv[j] - L.y1, dy2 = L.y2 - pyv[j];
-            const double e1 = ze - L.z1, e2 = L.z2 - ze;
-            const double A = __builtin_fma(e1, e1, __builtin_fma(dy1, dy1, dxa));
-            const double B = __builtin_fma(e2, e2, __builtin_fma(dy2, dy2, dxb));
-            const double os = __builtin_fma(n2, sqrt_est(B), n1 * sqrt_est(A));
-            if (L.screen) {
-                runS = fmin(runS, os);
-                M = fmin(M, (runS + L.D) * (1.0 + 3.0 * kRho));
-                if (os > M * (1.0 + 2.0 * kRho) + L.D) continue;
-            }
-            pass |= 1u << jj;
         }
         // exact evaluation of the survivors
         while (pass) {

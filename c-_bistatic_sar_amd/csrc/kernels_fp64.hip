@@ -27,12 +27,13 @@ __global__ __launch_bounds__(256) void k_dem_interp(DemDev dem, const double* __
     z[k] = dem_interp(dem, xy[2 * k], xy[2 * k + 1]);
 }
 
+template <bool kLds>
 __global__ __launch_bounds__(256, 2) void k_refraction(DemDev dem, const double* __restrict__ legs, int64_t m,
                                                     double n1, double n2, double* __restrict__ out)
 {
     extern __shared__ double lds[];
     DemDev d = dem;
-    if (dem.nx + dem.ny + dem.ny * dem.zs <= kLdsDemMax) {
+    if (kLds) {
         d = dem_to_lds(dem, lds);
         __syncthreads();
     }
@@ -51,13 +52,14 @@ __global__ __launch_bounds__(256, 2) void k_refraction(DemDev dem, const double*
 // threads, so a wavefront searches around one antenna position for
 // neighbouring points (coherent DEM accesses and screening outcomes); the
 // outputs are [V][N][3] as calculaSlantRangeFermat returns them per target.
+template <bool kLds>
 __global__ __launch_bounds__(256, 2) void k_slant_range(DemDev dem, Traj tr, const double* __restrict__ pts,
                                                      int64_t V, double n1, double n2,
                                                      double* __restrict__ ida, double* __restrict__ volta)
 {
     extern __shared__ double lds[];
     DemDev d = dem;
-    if (dem.nx + dem.ny + dem.ny * dem.zs <= kLdsDemMax) {
+    if (kLds) {
         d = dem_to_lds(dem, lds);
         __syncthreads();
     }
@@ -167,8 +169,11 @@ hipError_t launch_refraction(const DemDev& dem, const double* legs, int64_t m, d
                              double* out, hipStream_t s)
 {
     if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_refraction, dim3(blocks_for(m, 256)), dim3(256), dem_lds_bytes(dem), s,
-                       dem, legs, m, n1, n2, out);
+    const size_t lb = dem_lds_bytes(dem);
+    if (lb)
+        hipLaunchKernelGGL(k_refraction<true>, dim3(blocks_for(m, 256)), dim3(256), lb, s, dem, legs, m, n1, n2, out);
+    else
+        hipLaunchKernelGGL(k_refraction<false>, dim3(blocks_for(m, 256)), dim3(256), 0, s, dem, legs, m, n1, n2, out);
     return hipGetLastError();
 }
 
@@ -177,8 +182,13 @@ hipError_t launch_slant_range(const DemDev& dem, const Traj& tr, const double* p
 {
     const int64_t m = V * tr.n;
     if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_slant_range, dim3(blocks_for(m, 256)), dim3(256), dem_lds_bytes(dem), s,
-                       dem, tr, pts, V, n1, n2, ida, volta);
+    const size_t lb = dem_lds_bytes(dem);
+    if (lb)
+        hipLaunchKernelGGL(k_slant_range<true>, dim3(blocks_for(m, 256)), dim3(256), lb, s, dem, tr, pts, V, n1, n2,
+                           ida, volta);
+    else
+        hipLaunchKernelGGL(k_slant_range<false>, dim3(blocks_for(m, 256)), dim3(256), 0, s, dem, tr, pts, V, n1, n2,
+                           ida, volta);
     return hipGetLastError();
 }
 
