@@ -63,18 +63,25 @@ int upload_dem(bsar_ctx* ctx, const bsar_dem* dem, double** dX, double** dY, dou
     }
     *zmin = lo;
     *zmax = hi;
+    // Z in FP64, followed in the same allocation by its FP32 rounding (the
+    // refraction search's screening estimates read only the FP32 copy)
+    std::vector<float> Zf(Z.size());
+    for (size_t k = 0; k < Z.size(); ++k) Zf[k] = (float)Z[k];
     size_t bx = X.size() * sizeof(double), by = Y.size() * sizeof(double), bz = Z.size() * sizeof(double);
+    size_t bzf = Zf.size() * sizeof(float);
     BSAR_HIP(ctx, hipMalloc((void**)dX, bx));
     BSAR_HIP(ctx, hipMalloc((void**)dY, by));
-    BSAR_HIP(ctx, hipMalloc((void**)dZ, bz));
+    BSAR_HIP(ctx, hipMalloc((void**)dZ, bz + bzf));
     BSAR_HIP(ctx, hipMemcpyAsync(*dX, X.data(), bx, hipMemcpyHostToDevice, ctx->stream));
     BSAR_HIP(ctx, hipMemcpyAsync(*dY, Y.data(), by, hipMemcpyHostToDevice, ctx->stream));
     BSAR_HIP(ctx, hipMemcpyAsync(*dZ, Z.data(), bz, hipMemcpyHostToDevice, ctx->stream));
+    BSAR_HIP(ctx, hipMemcpyAsync((char*)*dZ + bz, Zf.data(), bzf, hipMemcpyHostToDevice, ctx->stream));
     BSAR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     DemDev d{};
     d.X = *dX;
     d.Y = *dY;
     d.Z = *dZ;
+    d.Zf = reinterpret_cast<const float*>((char*)*dZ + bz);
     d.nx = dem->nx;
     d.ny = dem->ny;
     d.zs = zs;
@@ -468,7 +475,8 @@ int bsar_dataset_create(bsar_ctx* c, const bsar_params* p, const bsar_dem* dem, 
     ds->traj.ry = ds->dtraj + 4 * (size_t)n;
     ds->traj.rz = ds->dtraj + 5 * (size_t)n;
     ds->traj.n = n;
-    ds->bytes = (int64_t)soa.size() * 8 + (int64_t)(ds->dem.nx + ds->dem.ny + ds->dem.ny * ds->dem.zs) * 8;
+    ds->bytes = (int64_t)soa.size() * 8 + (int64_t)(ds->dem.nx + ds->dem.ny + ds->dem.ny * ds->dem.zs) * 8 +
+                (int64_t)ds->dem.ny * ds->dem.zs * 4;
     *out = ds;
     return BSAR_OK;
 }

@@ -22,6 +22,7 @@ struct DemDev {
     const double* X;
     const double* Y;
     const double* Z;
+    const float* Zf;           // Z rounded to FP32, same layout (screening estimates only)
     int nx, ny, zs, flags;
     double x0, ihx, y0, ihy;   // first value and inverse mean spacing (index guess only)
     double zlo, zhi;           // height range of the stored Z (0, 0 when empty)
@@ -98,40 +99,58 @@ __device__ __forceinline__ double dem_interp(const DemDev& d, double x, double y
 //       n1*sqrt(dx1^2 + g1^2) + n2*sqrt(dx2^2 + g2^2)
 //     (horizontal offset along one axis, vertical gap g to the DEM height
 //     range) exceeds the best value known so far are skipped;
-//   * every other candidate gets a cheap estimate (FMA bilinear, sqrt as
-//     a * v_rsq_f64(a)) and is evaluated exactly only when the estimate,
-//     widened by its error bound, could reach the best value;
+//   * every other candidate gets a cheap FP32 estimate (bilinear height
+//     from the FP32 copy of Z, raw v_sqrt_f32) and is evaluated exactly only
+//     when the estimate, widened by its error bound, could reach the best
+//     value;
 //   * candidates are visited centre-out (the previous level's best sits at
 //     the centre), and ties between exact values resolve to the lowest
 //     reference index i*G + j -- the first one the sequential scan meets.
 //
-// Error bounds: v_rsq_f64 is within 5.3e-8 relative of 1/sqrt (measured on
-// gfx950); kRho = 2e-6 covers it with margin, together with the few
-// roundings of the estimate.  kDefl deflates the lower bounds by more than
-// their rounding.  Any candidate dropped by either test has an exact value
-// strictly above an exact value already reached (or above the minimum of the
-// previous levels), so it can neither be the minimum nor tie with it.
-constexpr double kRho = 2e-6;
+// Error bounds.  Lower bounds: v_rsq_f64 is within 5.3e-8 relative of
+// 1/sqrt (measured on gfx950); kDefl deflates them by more than that and
+// their few roundings.  Estimates: with u = 2^-24, the FP32 inputs (Z, the
+// bilinear weights, z1, z2, the squared horizontal offsets, n1, n2: one
+// rounding each), the bilinear blend (~30 u |Z|max absolute), the sums and
+// v_sqrt_f32 (1 ulp) give
+//   |est - exact| <= 13 u est + 3 (n1 + n2) 31 u (|Z|max + |z1| + |z2|),
+// i.e. below kRho32 * est + D32 with kRho32 = 4e-6 (5x) and
+// D32 = (n1 + n2) 1e-5 (|Z|max + |z1| + |z2| + 1) (1.6x).  A candidate is
+// dropped only when est > M (1 + 2 kRho32) + 2 D32, which puts its exact
+// value above M; M itself only takes exact values or est upper bounds
+// (est + D32)(1 + 3 kRho32).  Any candidate dropped by a bound or an estimate
+// has an exact value strictly above an exact value already reached (or above
+// the minimum of the previous levels), so it can neither be the minimum nor
+// tie with it.
+constexpr double kRho32 = 4e-6;
 constexpr double kDefl = 1.0 - 8e-6;
 
 __device__ __forceinline__ double sqrt_est(double a) { return a * __builtin_amdgcn_rsq(a); }
+__device__ __forceinline__ float sqrt_f32(float a) { return __builtin_amdgcn_sqrtf(a); }
 
 struct Leg {
     double x1, y1, z1, x2, y2, z2, n1, n2;
     double g1sq, g2sq;   // squared vertical gaps from z1 / z2 to the DEM height range
-    double D;            // absolute error bound of the estimate
-    bool screen;         // bounds valid (n1, n2 >= 0 and finite); else every candidate is exact
+    double D;            // absolute error bound of the FP32 estimate (D32)
+    float z1f, z2f, n1f, n2f;
+    bool screen;         // bounds valid (n1, n2 >= 0, finite, moderate); else every candidate is exact
 };
 
 __device__ __forceinline__ Leg make_leg(const DemDev& d, double x1, double y1, double z1, double x2, double y2,
                                         double z2, double n1, double n2)
 {
-    Leg L{x1, y1, z1, x2, y2, z2, n1, n2, 0.0, 0.0, 0.0, n1 >= 0.0 && n2 >= 0.0 && n1 + n2 < 1e300};
+    Leg L{};
+    L.x1 = x1; L.y1 = y1; L.z1 = z1; L.x2 = x2; L.y2 = y2; L.z2 = z2; L.n1 = n1; L.n2 = n2;
     const double g1 = fmax(fmax(d.zlo - z1, z1 - d.zhi), 0.0);
     const double g2 = fmax(fmax(d.zlo - z2, z2 - d.zhi), 0.0);
     L.g1sq = g1 * g1;
     L.g2sq = g2 * g2;
-    L.D = (fabs(n1) + fabs(n2)) * 1e-13 * (d.zabs + fabs(z1) + fabs(z2) + 1.0);
+    L.D = (fabs(n1) + fabs(n2)) * 1e-5 * (d.zabs + fabs(z1) + fabs(z2) + 1.0);
+    L.z1f = (float)z1; L.z2f = (float)z2; L.n1f = (float)n1; L.n2f = (float)n2;
+    // FP32 estimates need every magnitude well inside FP32 range
+    const double big = 1e15;
+    L.screen = n1 >= 0.0 && n2 >= 0.0 && n1 + n2 < 1e6 && fabs(x1) < big && fabs(y1) < big && fabs(z1) < big &&
+               fabs(x2) < big && fabs(y2) < big && fabs(z2) < big && d.zabs < big;
     return L;
 }
 
@@ -205,14 +224,15 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     // Per-column terms for the columns whose lower bound can still reach M;
     // jr[j] is the Z offset of row j0 with bit 0 set when the y interval was
     // found.  Pruned columns get lbc = +inf and are never read again.
+    // FP32 column terms: weight, squared y offsets to both ends.
     int jr[G];
-    double wy[G], pyv[G];
-    float lbc[G];
+    float wyf[G], sy1[G], sy2[G], lbc[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
         jr[j] = 0;
-        wy[j] = 0.0;
-        pyv[j] = 0.0;
+        wyf[j] = 0.0f;
+        sy1[j] = 0.0f;
+        sy2[j] = 0.0f;
         const double pya = __builtin_fma((double)j, stepY, minY);
         const double a = fmax(fabs(pya - L.y1) - epsy, 0.0), b = fmax(fabs(pya - L.y2) - epsy, 0.0);
         const double lb = (n1 * sqrt_est(__builtin_fma(a, a, L.g1sq)) + n2 * sqrt_est(__builtin_fma(b, b, L.g2sq))) * kDefl;
@@ -226,8 +246,10 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         double w = 0.0;
         if (!empty) axis_lookup(d.Y, d.ny, d.y0, d.ihy, ysorted, py, j0, dj, w);
         jr[j] = ((j0 * d.zs) << 1) | dj;
-        wy[j] = w;
-        pyv[j] = py;
+        wyf[j] = (float)w;
+        const double dy1 = py - L.y1, dy2 = L.y2 - py;
+        sy1[j] = (float)(dy1 * dy1);
+        sy2[j] = (float)(dy2 * dy2);
     }
 
 #pragma unroll 1
@@ -263,26 +285,26 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         unsigned pass = colok;
         if (L.screen) {
             pass = 0u;
-            const double* zr = d.Z + i0;
+            const float* zr = d.Zf + i0;
+            const float wxf = (float)wx, dxaf = (float)dxa, dxbf = (float)dxb;
 #pragma unroll
             for (int jj = 0; jj < G; ++jj) {
                 const int j = center_out(jj, C);
                 const int r0o = jr[j] >> 1;
                 const int r1o = r0o + ((jr[j] & 1) ? d.zs : 0);
-                const double z00 = zr[r0o], z01 = zr[r0o + di];
-                const double z10 = zr[r1o], z11 = zr[r1o + di];
-                const double za = __builtin_fma(z01 - z00, wx, z00);
-                const double zb = __builtin_fma(z11 - z10, wx, z10);
-                const double ze = __builtin_fma(zb - za, wy[j], za);
-                const double dy1 = pyv[j] - L.y1, dy2 = L.y2 - pyv[j];
-                const double e1 = ze - L.z1, e2 = L.z2 - ze;
-                const double A = __builtin_fma(e1, e1, __builtin_fma(dy1, dy1, dxa));
-                const double B = __builtin_fma(e2, e2, __builtin_fma(dy2, dy2, dxb));
-                const double os = __builtin_fma(n2, sqrt_est(B), n1 * sqrt_est(A));
+                const float z00 = zr[r0o], z01 = zr[r0o + di];
+                const float z10 = zr[r1o], z11 = zr[r1o + di];
+                const float za = __builtin_fmaf(z01 - z00, wxf, z00);
+                const float zb = __builtin_fmaf(z11 - z10, wxf, z10);
+                const float ze = __builtin_fmaf(zb - za, wyf[j], za);
+                const float e1 = ze - L.z1f, e2 = L.z2f - ze;
+                const float A = __builtin_fmaf(e1, e1, dxaf + sy1[j]);
+                const float B = __builtin_fmaf(e2, e2, dxbf + sy2[j]);
+                const double os = (double)__builtin_fmaf(L.n2f, sqrt_f32(B), L.n1f * sqrt_f32(A));
                 const bool ok = (colok >> jj) & 1u;
-                // M = min(M, (smallest estimate so far + D) (1 + 3 kRho)); NaN estimates leave it
-                M = fmin(M, ((ok ? os : __builtin_inf()) + L.D) * (1.0 + 3.0 * kRho));
-                pass |= (ok && !(os > M * (1.0 + 2.0 * kRho) + L.D)) ? (1u << jj) : 0u;
+                // M = min(M, upper bound of the best estimate so far); NaN estimates leave it
+                M = fmin(M, ((ok ? os : __builtin_inf()) + L.D) * (1.0 + 3.0 * kRho32));
+                pass |= (ok && !(os > M * (1.0 + 2.0 * kRho32) + 2.0 * L.D)) ? (1u << jj) : 0u;
             }
         }
         // exact evaluation of the survivors
@@ -327,10 +349,26 @@ __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double 
     qz = dem_interp(d, bestX, bestY);
 }
 
+// 8-byte LDS slots of a staged DEM: X, Y, Z (FP64), then Zf (FP32).
+__host__ __device__ __forceinline__ int dem_lds_slots(const DemDev& g)
+{
+    return g.nx + g.ny + g.ny * g.zs + (g.ny * g.zs + 1) / 2;
+}
+
+// View of a DEM staged at lds (no copy).
+__device__ __forceinline__ DemDev dem_lds_view(const DemDev& g, double* lds)
+{
+    DemDev d = g;
+    d.X = lds;
+    d.Y = lds + g.nx;
+    d.Z = lds + g.nx + g.ny;
+    d.Zf = reinterpret_cast<const float*>(lds + g.nx + g.ny + g.ny * g.zs);
+    return d;
+}
+
 // Cooperative copy of the DEM into LDS; returns a view onto the copy.
 __device__ __forceinline__ DemDev dem_to_lds(const DemDev& g, double* lds)
 {
-    DemDev d = g;
     const int nz = g.ny * g.zs;
     const int total = g.nx + g.ny + nz;
     for (int k = threadIdx.x; k < total; k += blockDim.x) {
@@ -340,10 +378,9 @@ __device__ __forceinline__ DemDev dem_to_lds(const DemDev& g, double* lds)
         else v = g.Z[k - g.nx - g.ny];
         lds[k] = v;
     }
-    d.X = lds;
-    d.Y = lds + g.nx;
-    d.Z = lds + g.nx + g.ny;
-    return d;
+    float* zf = reinterpret_cast<float*>(lds + total);
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) zf[k] = g.Zf[k];
+    return dem_lds_view(g, lds);
 }
 
 }  // namespace bsar

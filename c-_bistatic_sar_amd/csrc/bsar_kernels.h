@@ -7,8 +7,9 @@
 
 namespace bsar {
 
-// DEMs up to this many doubles (X + Y + padded Z) are staged in LDS per
-// workgroup; larger ones are read through the L1/L2 caches.
+// DEMs up to this many 8-byte slots (X + Y + padded Z in FP64, then the FP32
+// copy of Z) are staged in LDS per workgroup; larger ones are read through
+// the L1/L2 caches.
 constexpr int kLdsDemMax = 8192;   // 64 KiB
 
 // Antenna positions, structure of arrays, resident in HBM.

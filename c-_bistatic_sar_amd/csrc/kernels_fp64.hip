@@ -154,7 +154,7 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs
 
 static inline size_t dem_lds_bytes(const DemDev& d)
 {
-    const int cnt = d.nx + d.ny + d.ny * d.zs;
+    const int cnt = dem_lds_slots(d);
     return cnt <= kLdsDemMax ? (size_t)cnt * sizeof(double) : 0;
 }
 

@@ -103,7 +103,7 @@ hipError_t launch_bp_mixed(const BpArgs& a, const MixedArgs& m, hipStream_t s)
 {
     const int64_t threads = (int64_t)a.groups * a.nvox_pad;
     if (threads <= 0) return hipSuccess;
-    const int cnt = a.dem.nx + a.dem.ny + a.dem.ny * a.dem.zs;
+    const int cnt = dem_lds_slots(a.dem);
     if (cnt <= kLdsDemMax)
         hipLaunchKernelGGL(k_bp_mixed<true>, dim3(nb(threads, 256)), dim3(256), (size_t)cnt * sizeof(double), s, a, m);
     else

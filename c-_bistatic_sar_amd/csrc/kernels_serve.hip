@@ -37,9 +37,7 @@ __global__ __launch_bounds__(256, 2) void k_serve(const SrvJobDev* __restrict__ 
                 __syncthreads();         // staged, and every wave has read s_dem
                 if (threadIdx.x == 0) s_dem = J.a.dem.X;
             } else {
-                d.X = lds;
-                d.Y = lds + J.a.dem.nx;
-                d.Z = lds + J.a.dem.nx + J.a.dem.ny;
+                d = dem_lds_view(J.a.dem, lds);
             }
         }
         if ((int)threadIdx.x < w.nvox) {

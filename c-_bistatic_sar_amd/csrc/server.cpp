@@ -281,8 +281,7 @@ int setup(bsar_server* s, Job* j)
     } else {
         J.a.acc = (double2*)j->dmem;
     }
-    const int dem_doubles = j->ds->dem.nx + j->ds->dem.ny + j->ds->dem.ny * j->ds->dem.zs;
-    j->variant = 2 * (mixed ? 1 : 0) + (dem_doubles <= kSrvLdsDem ? 1 : 0);
+    j->variant = 2 * (mixed ? 1 : 0) + (dem_lds_slots(j->ds->dem) <= kSrvLdsDem ? 1 : 0);
     j->cost_vp = mixed ? bp_cost_mixed() : bp_cost_fp64(j->K);
     j->tiles = (j->V + 255) / 256;
     s->h_jobs[j->slot] = J;
