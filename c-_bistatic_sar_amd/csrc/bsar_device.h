@@ -156,6 +156,57 @@ __device__ __forceinline__ Leg make_leg(const DemDev& d, double x1, double y1, d
 
 __device__ __forceinline__ int center_out(int k, int c) { return (k & 1) ? c - ((k + 1) >> 1) : c + (k >> 1); }
 
+// Largest i with A[i] <= x (0 when none) and smallest i with A[i] >= x
+// (n - 1 when none) on a strictly increasing axis, from the uniform-spacing
+// guess.
+__device__ __forceinline__ int node_floor(const double* A, int n, double a0, double ih, double x)
+{
+    const double gd = (x - a0) * ih;
+    int i = !(gd > 0.0) ? 0 : (gd >= (double)(n - 1) ? n - 1 : (int)gd);
+    while (i > 0 && A[i] > x) --i;
+    while (i + 1 < n && A[i + 1] <= x) ++i;
+    return i;
+}
+__device__ __forceinline__ int node_ceil(const double* A, int n, double a0, double ih, double x)
+{
+    const double gd = (x - a0) * ih;
+    int i = !(gd > 0.0) ? 0 : (gd >= (double)(n - 1) ? n - 1 : (int)gd);
+    while (i + 1 < n && A[i] < x) ++i;
+    while (i > 0 && A[i - 1] >= x) --i;
+    return i;
+}
+
+// Squared vertical gaps from z1 / z2 to the heights the DEM lookup can return
+// inside the window [xa, xb] x [ya, yb]: convex combinations of the nodes of
+// the intervals the window meets, plus column / row 0 where it leaves the
+// axis range (raw_data_processing.cpp:621-671).  Left unchanged (the global
+// range) for unsorted axes or windows over more than 16 nodes.
+__device__ __forceinline__ void local_gaps(const DemDev& d, double z1, double z2, double xa, double xb, double ya,
+                                           double yb, double& g1sq, double& g2sq)
+{
+    const int i0 = node_floor(d.X, d.nx, d.x0, d.ihx, xa), i1 = node_ceil(d.X, d.nx, d.x0, d.ihx, xb);
+    const int j0 = node_floor(d.Y, d.ny, d.y0, d.ihy, ya), j1 = node_ceil(d.Y, d.ny, d.y0, d.ihy, yb);
+    if (i1 < i0 || j1 < j0 || (i1 - i0 + 1) * (j1 - j0 + 1) > 16) return;
+    const bool wx0 = xa < d.X[0] || xb > d.X[d.nx - 1];
+    const bool wy0 = ya < d.Y[0] || yb > d.Y[d.ny - 1];
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    for (int j = j0; j <= j1; ++j) {
+        const double* r = d.Z + j * d.zs;
+        for (int i = i0; i <= i1; ++i) { lo = fmin(lo, r[i]); hi = fmax(hi, r[i]); }
+        if (wx0) { lo = fmin(lo, r[0]); hi = fmax(hi, r[0]); }
+    }
+    if (wy0) {
+        for (int i = i0; i <= i1; ++i) { lo = fmin(lo, d.Z[i]); hi = fmax(hi, d.Z[i]); }
+        lo = fmin(lo, d.Z[0]);
+        hi = fmax(hi, d.Z[0]);
+    }
+    if (!(hi >= lo)) return;   // every node NaN: keep the global range
+    const double g1 = fmax(fmax(lo - z1, z1 - hi), 0.0);
+    const double g2 = fmax(fmax(lo - z2, z2 - hi), 0.0);
+    g1sq = g1 * g1;
+    g2sq = g2 * g2;
+}
+
 // Exact optical path of candidate (px, py) in the reference's operation order
 // (raw_data_processing.cpp:525-534 with interpolateDEMElevation :615-680);
 // i0, di, wx are px's x-interval terms, dxa = (px - x1)^2, dxb = (x2 - px)^2.
@@ -202,6 +253,10 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     const double stepX = spanX / (G - 1), stepY = spanY / (G - 1);
     const double epsx = 1e-12 * (fabs(minX) + fabs(spanX) + 1.0);
     const double epsy = 1e-12 * (fabs(minY) + fabs(spanY) + 1.0);
+    // vertical gaps for the bounds: the DEM's global height range, or the
+    // range inside this level's window once the window is small
+    double g1sq = L.g1sq, g2sq = L.g2sq;
+    if (G == 11 && !empty && xsorted && ysorted) local_gaps(d, L.z1, L.z2, minX, maxX, minY, maxY, g1sq, g2sq);
 
     // Probe: the centre candidate (the previous level's best) evaluated
     // exactly first, so the bounds below start from a tight value.
@@ -235,7 +290,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         sy2[j] = 0.0f;
         const double pya = __builtin_fma((double)j, stepY, minY);
         const double a = fmax(fabs(pya - L.y1) - epsy, 0.0), b = fmax(fabs(pya - L.y2) - epsy, 0.0);
-        const double lb = (n1 * sqrt_est(__builtin_fma(a, a, L.g1sq)) + n2 * sqrt_est(__builtin_fma(b, b, L.g2sq))) * kDefl;
+        const double lb = (n1 * sqrt_est(__builtin_fma(a, a, g1sq)) + n2 * sqrt_est(__builtin_fma(b, b, g2sq))) * kDefl;
         if (lb > M) {
             lbc[j] = __builtin_inff();
             continue;
@@ -260,8 +315,8 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
             const double pxa = __builtin_fma((double)i, stepX, minX);
             const double a1 = fmax(fabs(pxa - L.x1) - epsx, 0.0);
             const double a2 = fmax(fabs(pxa - L.x2) - epsx, 0.0);
-            const double lbr = (n1 * sqrt_est(__builtin_fma(a1, a1, L.g1sq)) +
-                                n2 * sqrt_est(__builtin_fma(a2, a2, L.g2sq))) * kDefl;
+            const double lbr = (n1 * sqrt_est(__builtin_fma(a1, a1, g1sq)) +
+                                n2 * sqrt_est(__builtin_fma(a2, a2, g2sq))) * kDefl;
             if (lbr > M) continue;
         }
         // columns whose lower bound can still reach M (bit jj <-> column
