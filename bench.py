@@ -125,6 +125,33 @@ def write_reference_inputs(proc: Path, sc, root: np.ndarray, gx, gy, gz) -> None
     (proc / "environment.txt").write_text("n1=%r\nn2=%r\n" % (sc.n1, sc.n2))
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_port_all_cores(sc, root, side: int = 10) -> dict:
+    """The C restatement (oracle/liboracle.so, the reference's per-voxel work
+    including its reflection-point scan) over every host core, on a strided
+    side^3 voxel subset."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    sub = lambda a, m: a[np.linspace(0, len(a) - 1, m).round().astype(int)]
+    gx, gy, gz = sub(sc.gx, side), sub(sc.gy, side), sub(sc.gz, side)
+    up = O.upsample(root, sc.upsample)
+    t0 = time.time()
+    O.backproject(sc.bp_params(), sc.dem, sc.tx, sc.rx, up, gx, gy, gz, reflection=True, threads=cores)
+    secs = time.time() - t0
+    return {"value": round(side ** 3 * sc.n / secs, 1), "unit": "voxel-pulse/s", "cores": cores, "kind": "port",
+            "sample": f"{side}x{side}x{side} strided voxels x {sc.n} pulses in {secs:.1f} s"}
+
+
 def cpu_baseline(sc, root, target_s: float = 20.0) -> dict | None:
     """The reference's own OpenMP back-projection (oracle/_ref/proc_test_script,
     built from the reference sources) on a strided voxel subset of the same
@@ -406,6 +433,8 @@ def run_wide(args, pkg, scene) -> dict:
         wall64 = time.perf_counter() - t1
         sub = img_m[:, sx, sx]
         err = float(np.abs(sub - img64).max() / np.abs(img64).max())
+        Q = load("bsar_amd.quality", PKG_DIR / "quality.py")
+        lob = Q.lobe_deltas(np.ascontiguousarray(sub), img64, gx, gy, gz, sc.scat[np.abs(sc.rcs) >= 0.999])
         v64 = len(gx) * len(gy) * len(gz)
         log(f"fp64 subset image: {rep64['seconds']:.2f} s ({v64 * N / rep64['seconds']:.3e} voxel-pulse/s); "
             f"mixed vs FP64 {20 * np.log10(max(err, 1e-300)):.1f} dB")
@@ -428,7 +457,10 @@ def run_wide(args, pkg, scene) -> dict:
            "fp64_subset": {"voxels": v64, "value": round(v64 * N / rep64["seconds"], 1),
                            "seconds": round(rep64["seconds"], 2)},
            "mixed_vs_fp64": {"peak_err_db": round(20 * np.log10(max(err, 1e-300)), 2),
-                             "speedup": round((V * N / rep["seconds"]) / (v64 * N / rep64["seconds"]), 3)}}
+                             "speedup": round((V * N / rep["seconds"]) / (v64 * N / rep64["seconds"]), 3),
+                             "pslr_islr": {"targets": lob["targets"], "grid": "64x64x64 subset",
+                                           "max_abs_d_pslr_db": lob["max_abs_d_pslr_db"],
+                                           "max_abs_d_islr_db": lob["max_abs_d_islr_db"]}}}
     return out
 
 
@@ -508,13 +540,27 @@ def main() -> int:
                            "roofline": roofline(rep64, N, "fp64", traffic_from_profiles("k_bp_fp64"))}
             out["mixed_vs_fp64"] = {"speedup": round(w64 / (elapsed / args.steps), 3),
                                     "peak_err_db": round(20 * np.log10(max(err, 1e-300)), 2)}
+            Q = load("bsar_amd.quality", PKG_DIR / "quality.py")
+            pts = sc.scat[np.abs(sc.rcs) >= 0.999]
+            lob = Q.lobe_deltas(img_m, img64, sc.gx, sc.gy, sc.gz, pts)
+            out["mixed_vs_fp64"]["pslr_islr"] = {
+                "targets": lob["targets"], "max_abs_d_pslr_db": lob["max_abs_d_pslr_db"],
+                "max_abs_d_islr_db": lob["max_abs_d_islr_db"],
+                "fp64_pslr_db": [round(r["pslr_db"], 2) for r in lob["per_target"]],
+                "fp64_islr_db": [round(r["islr_db"], 2) for r in lob["per_target"]]}
+            log(f"PSLR/ISLR deltas over {lob['targets']} point targets: max |dPSLR| "
+                f"{lob['max_abs_d_pslr_db']:.2e} dB, max |dISLR| {lob['max_abs_d_islr_db']:.2e} dB")
         ds.close()
     if not args.no_cpu:
         log("cpu baseline ...")
         cb = cpu_baseline(sc, root)
         out["cpu_baseline"] = cb
         if cb:
+            cb["cpu_model"] = cpu_model()
             out["vs_cpu"] = round(value / cb["value"], 1)
+            log(f"cpu baseline: {cb['value']:.3e} voxel-pulse/s on {cb['cores']} threads ({cb['kind']}); "
+                "C restatement on every core ...")
+            cb["all_cores"] = cpu_port_all_cores(sc, root)
     print(json.dumps(out), flush=True)
     return 0
 
