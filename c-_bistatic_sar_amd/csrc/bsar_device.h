@@ -276,36 +276,41 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         M = L.screen ? fmin(minOpt, lvMin) : __builtin_inf();
     }
 
-    // Per-column terms for the columns whose lower bound can still reach M;
-    // jr[j] is the Z offset of row j0 with bit 0 set when the y interval was
-    // found.  Pruned columns get lbc = +inf and are never read again.
-    // FP32 column terms: weight, squared y offsets to both ends.
-    int jr[G];
-    float wyf[G], sy1[G], sy2[G], lbc[G];
+    // Columns whose lower bound can reach M (bit jj <-> column center_out(jj),
+    // fixed for the level: M only decreases, so later rows may estimate a
+    // column the bound would now drop) and their FP32 terms: Z element
+    // offsets of the rows j0 and j0 + dj, weight, squared y offsets.
+    int o0[G], o1[G];
+    float wyf[G], sy1[G], sy2[G];
+    unsigned colmask = 0u;
 #pragma unroll
-    for (int j = 0; j < G; ++j) {
-        jr[j] = 0;
+    for (int jj = 0; jj < G; ++jj) {
+        const int j = center_out(jj, C);
+        o0[j] = 0;
+        o1[j] = 0;
         wyf[j] = 0.0f;
         sy1[j] = 0.0f;
         sy2[j] = 0.0f;
         const double pya = __builtin_fma((double)j, stepY, minY);
         const double a = fmax(fabs(pya - L.y1) - epsy, 0.0), b = fmax(fabs(pya - L.y2) - epsy, 0.0);
         const double lb = (n1 * sqrt_est(__builtin_fma(a, a, g1sq)) + n2 * sqrt_est(__builtin_fma(b, b, g2sq))) * kDefl;
-        if (lb > M) {
-            lbc[j] = __builtin_inff();
-            continue;
-        }
-        lbc[j] = (float)lb;   // rounding up by < 6e-8 relative stays inside the kDefl margin
+        if (lb > M) continue;
+        colmask |= 1u << jj;
         const double py = minY + j * spanY / (G - 1);
         int j0 = 0, dj = 0;
         double w = 0.0;
         if (!empty) axis_lookup(d.Y, d.ny, d.y0, d.ihy, ysorted, py, j0, dj, w);
-        jr[j] = ((j0 * d.zs) << 1) | dj;
+        o0[j] = j0 * d.zs;
+        o1[j] = (j0 + dj) * d.zs;
         wyf[j] = (float)w;
         const double dy1 = py - L.y1, dy2 = L.y2 - py;
         sy1[j] = (float)(dy1 * dy1);
         sy2[j] = (float)(dy2 * dy2);
     }
+    // FP32 screening constants (rounded to FP32: well inside the 2 kRho32 and
+    // D32 slack of the bounds)
+    const float k3 = (float)(1.0 + 3.0 * kRho32), k2 = (float)(1.0 + 2.0 * kRho32);
+    const float Dk3 = (float)(L.D * (1.0 + 3.0 * kRho32)), D2 = (float)(2.0 * L.D);
 
 #pragma unroll 1
     for (int k = 0; k < G; ++k) {
@@ -319,13 +324,8 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
                                 n2 * sqrt_est(__builtin_fma(a2, a2, g2sq))) * kDefl;
             if (lbr > M) continue;
         }
-        // columns whose lower bound can still reach M (bit jj <-> column
-        // center_out(jj)); the probe (i, j) = (C, C) is bit 0 of row C
-        unsigned colok = 0u;
-#pragma unroll
-        for (int jj = 0; jj < G; ++jj)
-            colok |= (!((double)lbc[center_out(jj, C)] > M)) ? (1u << jj) : 0u;
-        if (i == C) colok &= ~1u;
+        // the probe (i, j) = (C, C) is bit 0 of row C
+        const unsigned colok = (i == C) ? (colmask & ~1u) : colmask;
         if (!colok) continue;
         const double px = minX + i * spanX / (G - 1);
         int i0 = 0, di = 0;
@@ -334,33 +334,41 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         const double dx1 = px - L.x1, dx2 = L.x2 - px;
         const double dxa = dx1 * dx1, dxb = dx2 * dx2;
         // Screening pass, branch-free: every candidate of the row is
-        // estimated (loads unconditional: Z rows are padded, and an empty
-        // DEM reads its zero row), so the LDS loads of a row issue together.
-        // Bit jj of pass: candidate (i, center_out(jj)) may still be the minimum.
+        // estimated from the node pairs (i0, i0 + 1) of rows o0 / o1 (in
+        // bounds: i0 + 1 <= nx - 1, or the zero padding column), so the LDS
+        // loads of a row issue together.  When the x interval was not found
+        // (di = 0) the second node gets weight 0: a non-finite neighbour then
+        // makes the estimate NaN, which only sends the candidate to the exact
+        // evaluation.  Bit jj of pass: candidate (i, center_out(jj)) may
+        // still be the minimum.
         unsigned pass = colok;
         if (L.screen) {
             pass = 0u;
             const float* zr = d.Zf + i0;
-            const float wxf = (float)wx, dxaf = (float)dxa, dxbf = (float)dxb;
+            const float wxf = di ? (float)wx : 0.0f, dxaf = (float)dxa, dxbf = (float)dxb;
+            float Mf = (float)(M * (1.0 + 0x1p-22));   // >= M: an upper bound of the minimum
 #pragma unroll
             for (int jj = 0; jj < G; ++jj) {
                 const int j = center_out(jj, C);
-                const int r0o = jr[j] >> 1;
-                const int r1o = r0o + ((jr[j] & 1) ? d.zs : 0);
-                const float z00 = zr[r0o], z01 = zr[r0o + di];
-                const float z10 = zr[r1o], z11 = zr[r1o + di];
+                const float* p0 = zr + o0[j];
+                const float* p1 = zr + o1[j];
+                const float z00 = p0[0], z01 = p0[1];
+                const float z10 = p1[0], z11 = p1[1];
                 const float za = __builtin_fmaf(z01 - z00, wxf, z00);
                 const float zb = __builtin_fmaf(z11 - z10, wxf, z10);
                 const float ze = __builtin_fmaf(zb - za, wyf[j], za);
                 const float e1 = ze - L.z1f, e2 = L.z2f - ze;
                 const float A = __builtin_fmaf(e1, e1, dxaf + sy1[j]);
                 const float B = __builtin_fmaf(e2, e2, dxbf + sy2[j]);
-                const double os = (double)__builtin_fmaf(L.n2f, sqrt_f32(B), L.n1f * sqrt_f32(A));
+                const float os = __builtin_fmaf(L.n2f, sqrt_f32(B), L.n1f * sqrt_f32(A));
                 const bool ok = (colok >> jj) & 1u;
-                // M = min(M, upper bound of the best estimate so far); NaN estimates leave it
-                M = fmin(M, ((ok ? os : __builtin_inf()) + L.D) * (1.0 + 3.0 * kRho32));
-                pass |= (ok && !(os > M * (1.0 + 2.0 * kRho32) + 2.0 * L.D)) ? (1u << jj) : 0u;
+                // Mf = min(Mf, (est + D32)(1 + 3 kRho32)); NaN estimates leave it
+                const float t = __builtin_fmaf(ok ? os : __builtin_inff(), k3, Dk3);
+                Mf = t < Mf ? t : Mf;
+                const bool keep = !(os > __builtin_fmaf(Mf, k2, D2));
+                pass |= (ok & keep) ? (1u << jj) : 0u;
             }
+            M = fmin(M, (double)Mf);
         }
         // exact evaluation of the survivors
         while (pass) {
