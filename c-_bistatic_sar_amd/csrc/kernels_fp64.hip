@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void k_dem_interp(DemDev dem, const double* __
 }
 
 template <bool kLds>
-__global__ __launch_bounds__(256, 2) void k_refraction(DemDev dem, const double* __restrict__ legs, int64_t m,
+__global__ __launch_bounds__(256, kLds ? 3 : 2) void k_refraction(DemDev dem, const double* __restrict__ legs, int64_t m,
                                                     double n1, double n2, double* __restrict__ out)
 {
     extern __shared__ double lds[];
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256, 2) void k_refraction(DemDev dem, const double*
 // neighbouring points (coherent DEM accesses and screening outcomes); the
 // outputs are [V][N][3] as calculaSlantRangeFermat returns them per target.
 template <bool kLds>
-__global__ __launch_bounds__(256, 2) void k_slant_range(DemDev dem, Traj tr, const double* __restrict__ pts,
+__global__ __launch_bounds__(256, kLds ? 3 : 2) void k_slant_range(DemDev dem, Traj tr, const double* __restrict__ pts,
                                                      int64_t V, double n1, double n2,
                                                      double* __restrict__ ida, double* __restrict__ volta)
 {
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void k_spline(const double2* __restrict__ in, 
 // this launch's window in increasing order -- with one group that is exactly
 // the reference's sequential accumulation (proc_test_script.cpp:589-641).
 template <bool kLds>
-__global__ __launch_bounds__(256, 2) void k_bp_fp64(BpArgs a)
+__global__ __launch_bounds__(256, kLds ? 3 : 2) void k_bp_fp64(BpArgs a)
 {
     extern __shared__ double lds[];
     DemDev d = a.dem;
