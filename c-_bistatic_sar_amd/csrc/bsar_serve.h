@@ -36,9 +36,10 @@ struct SrvList {
 
 constexpr int kSrvListCap = 32768;   // items per round
 
-// DEM doubles staged in LDS by the serving kernel (its dynamic LDS is fixed
-// at capture time; the margin leaves room for its static LDS within 64 KiB).
-constexpr int kSrvLdsDem = kLdsDemMax - 32;
+// DEM slots (8 bytes) staged in LDS by the serving kernel.  Its dynamic LDS
+// is fixed at capture time, so it is sized for four resident workgroups per
+// CU (4 x 36 KiB of the 160 KiB); larger DEMs take the global-memory variant.
+constexpr int kSrvLdsDem = 4608;
 inline size_t srv_list_bytes() { return sizeof(SrvList) + (kSrvListCap - 1) * sizeof(SrvItem); }
 
 const void* serve_kernel(int variant);

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void k_spline(const double2* __restrict__ in, 
 // this launch's window in increasing order -- with one group that is exactly
 // the reference's sequential accumulation (proc_test_script.cpp:589-641).
 template <bool kLds>
-__global__ __launch_bounds__(256, kLds ? 3 : 2) void k_bp_fp64(BpArgs a)
+__global__ __launch_bounds__(256, kLds ? 4 : 2) void k_bp_fp64(BpArgs a)
 {
     extern __shared__ double lds[];
     DemDev d = a.dem;

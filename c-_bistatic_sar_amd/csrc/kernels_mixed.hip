@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void k_r1_table(R1Table tab, const double* __r
 }
 
 template <bool kLds>
-__global__ __launch_bounds__(256, kLds ? 3 : 2) void k_bp_mixed(BpArgs a, MixedArgs m)
+__global__ __launch_bounds__(256, kLds ? 4 : 2) void k_bp_mixed(BpArgs a, MixedArgs m)
 {
     extern __shared__ double lds[];
     DemDev d = a.dem;

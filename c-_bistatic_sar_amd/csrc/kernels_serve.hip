@@ -14,7 +14,7 @@
 namespace bsar {
 
 template <bool kMixed, bool kLds>
-__global__ __launch_bounds__(256, 2) void k_serve(const SrvJobDev* __restrict__ jobs, SrvList* __restrict__ wl)
+__global__ __launch_bounds__(256, kLds ? 4 : 2) void k_serve(const SrvJobDev* __restrict__ jobs, SrvList* __restrict__ wl)
 {
     constexpr int kVar = 2 * (int)kMixed + (int)kLds;
     extern __shared__ double lds[];
