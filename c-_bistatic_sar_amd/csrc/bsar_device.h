@@ -429,6 +429,28 @@ __device__ __forceinline__ DemDev dem_lds_view(const DemDev& g, double* lds)
     return d;
 }
 
+// Cooperative copy of the search's hot part of a DEM into LDS: X, Y (FP64)
+// and the FP32 heights; the FP64 heights (read by exact evaluations only)
+// stay in global memory.  Slots: nx + ny + ceil(ny * zs / 2).
+__host__ __device__ __forceinline__ int dem_split_slots(const DemDev& g)
+{
+    return g.nx + g.ny + (g.ny * g.zs + 1) / 2;
+}
+
+__device__ __forceinline__ DemDev dem_to_lds_split(const DemDev& g, double* lds)
+{
+    const int nxy = g.nx + g.ny;
+    for (int k = threadIdx.x; k < nxy; k += blockDim.x) lds[k] = (k < g.nx) ? g.X[k] : g.Y[k - g.nx];
+    float* zf = reinterpret_cast<float*>(lds + nxy);
+    const int nz = g.ny * g.zs;
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) zf[k] = g.Zf[k];
+    DemDev d = g;
+    d.X = lds;
+    d.Y = lds + g.nx;
+    d.Zf = zf;
+    return d;
+}
+
 // Cooperative copy of the DEM into LDS; returns a view onto the copy.
 __device__ __forceinline__ DemDev dem_to_lds(const DemDev& g, double* lds)
 {
@@ -444,6 +466,30 @@ __device__ __forceinline__ DemDev dem_to_lds(const DemDev& g, double* lds)
     float* zf = reinterpret_cast<float*>(lds + total);
     for (int k = threadIdx.x; k < nz; k += blockDim.x) zf[k] = g.Zf[k];
     return dem_lds_view(g, lds);
+}
+
+}  // namespace bsar
+
+namespace bsar {
+
+// Where a kernel finds the DEM (template argument of the search kernels).
+enum DemMode : int { DEM_GLOBAL = 0, DEM_LDS = 1, DEM_SPLIT = 2 };
+
+// Stages the DEM for mode kMode (all threads of the block call it).
+template <int kMode>
+__device__ __forceinline__ DemDev dem_stage(const DemDev& g, double* lds)
+{
+    if (kMode == DEM_LDS) {
+        const DemDev d = dem_to_lds(g, lds);
+        __syncthreads();
+        return d;
+    }
+    if (kMode == DEM_SPLIT) {
+        const DemDev d = dem_to_lds_split(g, lds);
+        __syncthreads();
+        return d;
+    }
+    return g;
 }
 
 }  // namespace bsar

@@ -8,9 +8,50 @@
 namespace bsar {
 
 // DEMs up to this many 8-byte slots (X + Y + padded Z in FP64, then the FP32
-// copy of Z) are staged in LDS per workgroup; larger ones are read through
-// the L1/L2 caches.
+// copy of Z) are staged in LDS per workgroup (DEM_LDS); up to kSplitMax
+// slots of X, Y and the FP32 heights alone are staged with two workgroups per
+// CU, the FP64 heights read from global memory (DEM_SPLIT); larger DEMs are
+// read through the L1/L2 caches (DEM_GLOBAL).
 constexpr int kLdsDemMax = 8192;   // 64 KiB
+constexpr int kSplitMax = 9728;    // 76 KiB
+
+struct DemPlan {
+    int mode;       // DemMode
+    size_t bytes;   // dynamic LDS
+};
+inline DemPlan plan_dem(const DemDev& d)
+{
+    if (dem_lds_slots(d) <= kLdsDemMax) return {DEM_LDS, (size_t)dem_lds_slots(d) * 8};
+    if (dem_split_slots(d) <= kSplitMax) return {DEM_SPLIT, (size_t)dem_split_slots(d) * 8};
+    return {DEM_GLOBAL, 0};
+}
+
+// Waves per SIMD a search kernel is compiled for: `waves` with the whole DEM
+// in LDS, 2 otherwise (split mode holds two 256-thread workgroups per CU in
+// LDS; measured faster than larger workgroups at 3 waves).  Workgroups are
+// 256 threads in every mode.
+constexpr int dem_block(int, int) { return 256; }
+constexpr int dem_waves(int mode, int waves) { return mode == DEM_LDS ? waves : 2; }
+
+inline unsigned dem_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Launches kernel template K<mode> (W waves per SIMD with the DEM in LDS)
+// over n threads with the block size and dynamic LDS of plan.mode.
+#define BSAR_LAUNCH_DEM(K, W, plan, n, s, ...)                                                                  \
+    do {                                                                                                      \
+        if ((plan).mode == DEM_LDS)                                                                            \
+            hipLaunchKernelGGL(K<DEM_LDS>, dim3(dem_blocks(n, dem_block(DEM_LDS, W))),                          \
+                               dim3(dem_block(DEM_LDS, W)), (plan).bytes, s, __VA_ARGS__);                      \
+        else if ((plan).mode == DEM_SPLIT) {                                                                   \
+            (void)hipFuncSetAttribute((const void*)K<DEM_SPLIT>, hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                                      (int)(plan).bytes);                                                     \
+            hipLaunchKernelGGL(K<DEM_SPLIT>, dim3(dem_blocks(n, dem_block(DEM_SPLIT, W))),                      \
+                               dim3(dem_block(DEM_SPLIT, W)), (plan).bytes, s, __VA_ARGS__);                    \
+        }                                                                                                     \
+        else                                                                                                  \
+            hipLaunchKernelGGL(K<DEM_GLOBAL>, dim3(dem_blocks(n, dem_block(DEM_GLOBAL, W))),                    \
+                               dim3(dem_block(DEM_GLOBAL, W)), 0, s, __VA_ARGS__);                             \
+    } while (0)
 
 // Antenna positions, structure of arrays, resident in HBM.
 struct Traj {

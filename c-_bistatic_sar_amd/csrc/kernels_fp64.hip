@@ -27,16 +27,12 @@ __global__ __launch_bounds__(256) void k_dem_interp(DemDev dem, const double* __
     z[k] = dem_interp(dem, xy[2 * k], xy[2 * k + 1]);
 }
 
-template <bool kLds>
-__global__ __launch_bounds__(256, kLds ? 3 : 2) void k_refraction(DemDev dem, const double* __restrict__ legs, int64_t m,
-                                                    double n1, double n2, double* __restrict__ out)
+template <int kMode>
+__global__ __launch_bounds__(dem_block(kMode, 3), dem_waves(kMode, 3)) void k_refraction(
+    DemDev dem, const double* __restrict__ legs, int64_t m, double n1, double n2, double* __restrict__ out)
 {
     extern __shared__ double lds[];
-    DemDev d = dem;
-    if (kLds) {
-        d = dem_to_lds(dem, lds);
-        __syncthreads();
-    }
+    const DemDev d = dem_stage<kMode>(dem, lds);
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= m) return;
     const double* s = legs + 6 * k;
@@ -52,17 +48,13 @@ __global__ __launch_bounds__(256, kLds ? 3 : 2) void k_refraction(DemDev dem, co
 // threads, so a wavefront searches around one antenna position for
 // neighbouring points (coherent DEM accesses and screening outcomes); the
 // outputs are [V][N][3] as calculaSlantRangeFermat returns them per target.
-template <bool kLds>
-__global__ __launch_bounds__(256, kLds ? 3 : 2) void k_slant_range(DemDev dem, Traj tr, const double* __restrict__ pts,
-                                                     int64_t V, double n1, double n2,
-                                                     double* __restrict__ ida, double* __restrict__ volta)
+template <int kMode>
+__global__ __launch_bounds__(dem_block(kMode, 3), dem_waves(kMode, 3)) void k_slant_range(
+    DemDev dem, Traj tr, const double* __restrict__ pts, int64_t V, double n1, double n2, double* __restrict__ ida,
+    double* __restrict__ volta)
 {
     extern __shared__ double lds[];
-    DemDev d = dem;
-    if (kLds) {
-        d = dem_to_lds(dem, lds);
-        __syncthreads();
-    }
+    const DemDev d = dem_stage<kMode>(dem, lds);
     int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= V * tr.n) return;
     const int i = (int)(k / V);
@@ -110,15 +102,11 @@ __global__ __launch_bounds__(256) void k_spline(const double2* __restrict__ in, 
 // the running sum of its voxel across launches (acc), adding the pulses of
 // this launch's window in increasing order -- with one group that is exactly
 // the reference's sequential accumulation (proc_test_script.cpp:589-641).
-template <bool kLds>
-__global__ __launch_bounds__(256, kLds ? 4 : 2) void k_bp_fp64(BpArgs a)
+template <int kMode>
+__global__ __launch_bounds__(dem_block(kMode, 4), dem_waves(kMode, 4)) void k_bp_fp64(BpArgs a)
 {
     extern __shared__ double lds[];
-    DemDev d = a.dem;
-    if (kLds) {
-        d = dem_to_lds(a.dem, lds);
-        __syncthreads();
-    }
+    const DemDev d = dem_stage<kMode>(a.dem, lds);
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     // nvox_pad is a multiple of the wavefront size: g is wave-uniform.
     const int g = __builtin_amdgcn_readfirstlane((int)(t / a.nvox_pad));
@@ -152,11 +140,6 @@ __global__ __launch_bounds__(256) void k_group_sum(const double2* acc, int group
 
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-static inline size_t dem_lds_bytes(const DemDev& d)
-{
-    const int cnt = dem_lds_slots(d);
-    return cnt <= kLdsDemMax ? (size_t)cnt * sizeof(double) : 0;
-}
 
 hipError_t launch_dem_interp(const DemDev& dem, const double* xy, int64_t m, double* z, hipStream_t s)
 {
@@ -169,11 +152,8 @@ hipError_t launch_refraction(const DemDev& dem, const double* legs, int64_t m, d
                              double* out, hipStream_t s)
 {
     if (m <= 0) return hipSuccess;
-    const size_t lb = dem_lds_bytes(dem);
-    if (lb)
-        hipLaunchKernelGGL(k_refraction<true>, dim3(blocks_for(m, 256)), dim3(256), lb, s, dem, legs, m, n1, n2, out);
-    else
-        hipLaunchKernelGGL(k_refraction<false>, dim3(blocks_for(m, 256)), dim3(256), 0, s, dem, legs, m, n1, n2, out);
+    const DemPlan p = plan_dem(dem);
+    BSAR_LAUNCH_DEM(k_refraction, 3, p, m, s, dem, legs, m, n1, n2, out);
     return hipGetLastError();
 }
 
@@ -182,13 +162,8 @@ hipError_t launch_slant_range(const DemDev& dem, const Traj& tr, const double* p
 {
     const int64_t m = V * tr.n;
     if (m <= 0) return hipSuccess;
-    const size_t lb = dem_lds_bytes(dem);
-    if (lb)
-        hipLaunchKernelGGL(k_slant_range<true>, dim3(blocks_for(m, 256)), dim3(256), lb, s, dem, tr, pts, V, n1, n2,
-                           ida, volta);
-    else
-        hipLaunchKernelGGL(k_slant_range<false>, dim3(blocks_for(m, 256)), dim3(256), 0, s, dem, tr, pts, V, n1, n2,
-                           ida, volta);
+    const DemPlan p = plan_dem(dem);
+    BSAR_LAUNCH_DEM(k_slant_range, 3, p, m, s, dem, tr, pts, V, n1, n2, ida, volta);
     return hipGetLastError();
 }
 
@@ -204,9 +179,8 @@ hipError_t launch_bp_fp64(const BpArgs& a, hipStream_t s)
 {
     const int64_t threads = (int64_t)a.groups * a.nvox_pad;
     if (threads <= 0) return hipSuccess;
-    const size_t lds = dem_lds_bytes(a.dem);
-    if (lds) hipLaunchKernelGGL(k_bp_fp64<true>, dim3(blocks_for(threads, 256)), dim3(256), lds, s, a);
-    else hipLaunchKernelGGL(k_bp_fp64<false>, dim3(blocks_for(threads, 256)), dim3(256), 0, s, a);
+    const DemPlan p = plan_dem(a.dem);
+    BSAR_LAUNCH_DEM(k_bp_fp64, 4, p, threads, s, a);
     return hipGetLastError();
 }
 

@@ -42,15 +42,11 @@ __global__ __launch_bounds__(256) void k_r1_table(R1Table tab, const double* __r
     out[t] = s / (double)n;
 }
 
-template <bool kLds>
-__global__ __launch_bounds__(256, kLds ? 4 : 2) void k_bp_mixed(BpArgs a, MixedArgs m)
+template <int kMode>
+__global__ __launch_bounds__(dem_block(kMode, 4), dem_waves(kMode, 4)) void k_bp_mixed(BpArgs a, MixedArgs m)
 {
     extern __shared__ double lds[];
-    DemDev d = a.dem;
-    if (kLds) {
-        d = dem_to_lds(a.dem, lds);
-        __syncthreads();
-    }
+    const DemDev d = dem_stage<kMode>(a.dem, lds);
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int g = __builtin_amdgcn_readfirstlane((int)(t / a.nvox_pad));
     const int64_t vl = t - (int64_t)g * a.nvox_pad;
@@ -103,11 +99,8 @@ hipError_t launch_bp_mixed(const BpArgs& a, const MixedArgs& m, hipStream_t s)
 {
     const int64_t threads = (int64_t)a.groups * a.nvox_pad;
     if (threads <= 0) return hipSuccess;
-    const int cnt = dem_lds_slots(a.dem);
-    if (cnt <= kLdsDemMax)
-        hipLaunchKernelGGL(k_bp_mixed<true>, dim3(nb(threads, 256)), dim3(256), (size_t)cnt * sizeof(double), s, a, m);
-    else
-        hipLaunchKernelGGL(k_bp_mixed<false>, dim3(nb(threads, 256)), dim3(256), 0, s, a, m);
+    const DemPlan p = plan_dem(a.dem);
+    BSAR_LAUNCH_DEM(k_bp_mixed, 4, p, threads, s, a, m);
     return hipGetLastError();
 }
 
