@@ -21,7 +21,6 @@ import json
 import os
 import re
 import shutil
-import struct
 import subprocess
 import sys
 import tempfile
@@ -99,32 +98,6 @@ def traffic_from_profiles(kernel: str):
     return None
 
 
-def write_reference_inputs(proc: Path, sc, root: np.ndarray, gx, gy, gz) -> None:
-    """build/output/proc/ file set in the reference's formats (Appendix of
-    data_export.cpp: binary matrix header + CSVs written %.6f)."""
-    proc.mkdir(parents=True, exist_ok=True)
-    N, S = root.shape
-    with open(proc / "rootData_complex.bin", "wb") as f:
-        f.write(struct.pack("<III", 2, 1, 6) + b"double".ljust(16, b"\0") + struct.pack("<QQ", S, N))
-        f.write(np.ascontiguousarray(root).tobytes())
-
-    def vec(name, v):
-        (proc / name).write_text(",".join("%.6f" % x for x in v) + "\n")
-
-    def mat(name, m):
-        (proc / name).write_text("".join(",".join("%.6f" % x for x in row) + "\n" for row in m))
-
-    vec("grid_xAxis.csv", gx)
-    vec("grid_yAxis.csv", gy)
-    vec("grid_zAxis.csv", gz)
-    vec("DEM_X_vec.csv", sc.dem[0])
-    vec("DEM_Y_vec.csv", sc.dem[1])
-    mat("DEM_Z_matrix.csv", sc.dem[2])
-    mat("Tx_positions.csv", sc.tx)
-    mat("Rx_positions.csv", sc.rx)
-    (proc / "environment.txt").write_text("n1=%r\nn2=%r\n" % (sc.n1, sc.n2))
-
-
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -167,7 +140,8 @@ def cpu_baseline(sc, root, target_s: float = 20.0) -> dict | None:
     if ref.exists():
         work = Path(tempfile.mkdtemp(prefix="bsar_cpu_"))
         try:
-            write_reference_inputs(work / "build" / "output" / "proc", sc, root, gx, gy, gz)
+            load("bsar_amd.scene", PKG_DIR / "scene.py").write_reference_inputs(
+                work / "build" / "output" / "proc", sc, root, gx, gy, gz)
             threads = min(int(os.environ.get("OMP_NUM_THREADS", cores)), cores)
             env = dict(os.environ, OMP_NUM_THREADS=str(threads))
             t0 = time.time()

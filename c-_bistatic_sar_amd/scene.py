@@ -311,3 +311,39 @@ def matched_filter(scene: Scene) -> np.ndarray:
         sys.modules["bsar_amd"] = pkg
         spec.loader.exec_module(pkg)
     return reference_signal(pkg.chirp(-scene.bw / 2, scene.bw / 2, scene.T, scene.fs), scene.S)
+
+
+def write_reference_inputs(proc, sc: Scene, root: np.ndarray, gx=None, gy=None, gz=None) -> None:
+    """The build/output/proc/ file set that proc_test_script.cpp reads
+    (:225-292), in the reference's formats: rootData_complex.bin as
+    data_export.cpp:71-138 writes a 2-D complex matrix (header + payload, the
+    payload range-fastest: pulse p's samples at r + p*S, proc_test_script.cpp
+    :330), CSVs with six decimals (data_export.cpp:148,169), environment.txt
+    as raw_data_processing.cpp:14-46 parses it."""
+    import struct
+    from pathlib import Path
+    proc = Path(proc)
+    proc.mkdir(parents=True, exist_ok=True)
+    gx = sc.gx if gx is None else gx
+    gy = sc.gy if gy is None else gy
+    gz = sc.gz if gz is None else gz
+    N, S = root.shape
+    with open(proc / "rootData_complex.bin", "wb") as f:
+        f.write(struct.pack("<III", 2, 1, 6) + b"double".ljust(16, b"\0") + struct.pack("<QQ", S, N))
+        f.write(np.ascontiguousarray(root, dtype=np.complex128).tobytes())
+
+    def vec(name, v):
+        (proc / name).write_text(",".join("%.6f" % x for x in v) + "\n")
+
+    def mat(name, m):
+        (proc / name).write_text("".join(",".join("%.6f" % x for x in row) + "\n" for row in m))
+
+    vec("grid_xAxis.csv", gx)
+    vec("grid_yAxis.csv", gy)
+    vec("grid_zAxis.csv", gz)
+    vec("DEM_X_vec.csv", sc.dem[0])
+    vec("DEM_Y_vec.csv", sc.dem[1])
+    mat("DEM_Z_matrix.csv", sc.dem[2])
+    mat("Tx_positions.csv", sc.tx)
+    mat("Rx_positions.csv", sc.rx)
+    (proc / "environment.txt").write_text("n1=%r\nn2=%r\n" % (sc.n1, sc.n2))

@@ -100,3 +100,39 @@ def test_facade_demo_rejects_malformed_input(tmp_path):
     for bad in ("np.float64(1.0) 2\n", "3 3\n1 2\n", "2 2\n0 1\n0 1\n0 0\n0 0\n999999999\n"):
         r = subprocess.run([str(exe)], input=bad, capture_output=True, text=True, timeout=60)
         assert r.returncode == 2
+
+
+def test_scene_file_set_parses(tmp_path, scene_mod):
+    """scene.write_reference_inputs (behind bsar_sim) writes a file set that
+    bsar_proc reads back with the stated sizes."""
+    sc = scene_mod.make_scene("small", n_pulses=12, grid=(5, 4, 3), samples=256)
+    root = np.random.default_rng(2).normal(size=(12, 256)) + 1j * np.random.default_rng(3).normal(size=(12, 256))
+    scene_mod.write_reference_inputs(tmp_path / "build" / "output" / "proc", sc, root)
+    r = run_tool("--check", cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "rootData 256 x 12" in r.stdout and "grid 5 x 4 x 3" in r.stdout and "12 positions" in r.stdout
+
+
+@pytest.mark.gpu
+def test_sim_then_proc_round_trip(tmp_path, bsar, oracle):
+    """bsar_sim writes a workload's file set (GPU forward model); bsar_proc
+    images it; the result equals the oracle run on the same files' contents."""
+    r = subprocess.run(["python", str(PKG_DIR / "sim.py"), "--workload", "small", "--out", str(tmp_path),
+                        "--pulses", "16", "--grid", "6", "5", "4", "--samples", "512"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    r = run_tool("--fc", "1e9", "--fs", "800e6", cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    proc = tmp_path / "build" / "output" / "proc"
+    img = parse_image((proc / "output_3D_matrix.bin").read_bytes())
+    rd = lambda n: np.array([[float(x) for x in ln.split(",")] for ln in (proc / n).read_text().split()])
+    root, dims = bsar.read_matrix(proc / "rootData_complex.bin")
+    S, N = dims
+    gx, gy, gz = rd("grid_xAxis.csv")[0], rd("grid_yAxis.csv")[0], rd("grid_zAxis.csv")[0]
+    dem = (rd("DEM_X_vec.csv")[0], rd("DEM_Y_vec.csv")[0], rd("DEM_Z_matrix.csv"))
+    tx, rx = rd("Tx_positions.csv"), rd("Rx_positions.csv")
+    env = dict(ln.split("=") for ln in (proc / "environment.txt").read_text().split())
+    bp = dict(c=3e8, fs4=800e6 * 4, lam=3e8 / 1e9, n1=float(env["n1"]), n2=float(env["n2"]))
+    want = oracle.backproject(bp, dem, tx, rx, oracle.upsample(root.reshape(N, S)), gx, gy, gz)
+    assert np.abs(img - want).max() <= 1e-9 * np.abs(want).max()
+    assert np.abs(want).max() > 0
