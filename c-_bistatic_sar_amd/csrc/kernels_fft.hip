@@ -26,9 +26,20 @@ __device__ __forceinline__ void twiddle(double a, double& s, double& c) { sincos
 
 __device__ __forceinline__ uint32_t bitrev(uint32_t k, int logS) { return __brev(k) >> (32 - logS); }
 
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b) { return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename V>
+__device__ __forceinline__ V cadd(V a, V b) { return V{a.x + b.x, a.y + b.y}; }
+template <typename V>
+__device__ __forceinline__ V csub(V a, V b) { return V{a.x - b.x, a.y - b.y}; }
+
 // Fused correlation: out[c] = IFFT(FFT(in[c]) * H) for one column c per block.
-// sign = -1 forward.  H is the (already conjugated when correlating) reference
-// spectrum, length S, natural order.
+// H is the (already conjugated when correlating) reference spectrum, length
+// S, natural order.  The column is loaded bit-reversed, transformed by
+// radix-2 DIT stages taken two at a time (one LDS round trip and one
+// sin/cos per 4 elements: with w' = exp(-i pi j / 2h) the stage-s twiddle is
+// w'^2 and the second stage-(s+1) twiddle -i w'), multiplied by H, inverse
+// transformed by DIF stages two at a time, and stored bit-reversed / S.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_fft_filter_lds(const typename C2<T>::type* __restrict__ in,
                                                          typename C2<T>::type* __restrict__ out,
@@ -42,40 +53,67 @@ __global__ __launch_bounds__(1024) void k_fft_filter_lds(const typename C2<T>::t
     const V* src = in + col * S;
     for (int k = threadIdx.x; k < S; k += blockDim.x) x[bitrev(k, logS)] = src[k];
     __syncthreads();
-    // forward DIT, natural-order result
-    for (int s = 1; s <= logS; ++s) {
-        const int half = 1 << (s - 1);
+    // forward DIT; an odd stage count starts with the twiddle-free stage 1
+    int s = 1;
+    if (logS & 1) {
         for (int b = threadIdx.x; b < S / 2; b += blockDim.x) {
-            const int j = b & (half - 1);
-            const int i0 = ((b >> (s - 1)) << s) + j;
-            const int i1 = i0 + half;
+            const V u = x[2 * b], v = x[2 * b + 1];
+            x[2 * b] = cadd(u, v);
+            x[2 * b + 1] = csub(u, v);
+        }
+        __syncthreads();
+        s = 2;
+    }
+    for (; s < logS; s += 2) {                       // stages s, s + 1; h = 2^(s-1)
+        const int h = 1 << (s - 1);
+        for (int q = threadIdx.x; q < S / 4; q += blockDim.x) {
+            const int j = q & (h - 1);
+            const int e0 = ((q >> (s - 1)) << (s + 1)) + j;
             T sn, cs;
-            twiddle((T)(-(T)j / (T)half), sn, cs);          // exp(-i*pi*j/half)
-            const V u = x[i0], w = x[i1];
-            const V v = {w.x * cs - w.y * sn, w.x * sn + w.y * cs};
-            x[i0] = V{u.x + v.x, u.y + v.y};
-            x[i1] = V{u.x - v.x, u.y - v.y};
+            twiddle((T)(-(T)j / (T)(2 * h)), sn, cs);
+            const V w1 = V{cs, sn};                  // exp(-i pi j / 2h)
+            const V w0 = cmul(w1, w1);               // exp(-i pi j / h)
+            const V w2 = V{sn, -cs};                 // -i w1
+            const V x0 = x[e0], x1 = x[e0 + h], x2 = x[e0 + 2 * h], x3 = x[e0 + 3 * h];
+            const V t1 = cmul(x1, w0), t3 = cmul(x3, w0);
+            const V a0 = cadd(x0, t1), a1 = csub(x0, t1), a2 = cadd(x2, t3), a3 = csub(x2, t3);
+            const V u2 = cmul(a2, w1), u3 = cmul(a3, w2);
+            x[e0] = cadd(a0, u2);
+            x[e0 + 2 * h] = csub(a0, u2);
+            x[e0 + h] = cadd(a1, u3);
+            x[e0 + 3 * h] = csub(a1, u3);
         }
         __syncthreads();
     }
-    for (int k = threadIdx.x; k < S; k += blockDim.x) {
-        const V a = x[k], h = H[k];
-        x[k] = V{a.x * h.x - a.y * h.y, a.x * h.y + a.y * h.x};
-    }
+    for (int k = threadIdx.x; k < S; k += blockDim.x) x[k] = cmul(x[k], H[k]);
     __syncthreads();
-    // inverse DIF, bit-reversed result
-    for (int s = logS; s >= 1; --s) {
-        const int half = 1 << (s - 1);
-        for (int b = threadIdx.x; b < S / 2; b += blockDim.x) {
-            const int j = b & (half - 1);
-            const int i0 = ((b >> (s - 1)) << s) + j;
-            const int i1 = i0 + half;
+    // inverse DIF, stages taken two at a time from the top; an odd count
+    // ends with the twiddle-free stage 1
+    for (s = logS; s >= 2; s -= 2) {                 // stages s, s - 1; h = 2^(s-2)
+        const int h = 1 << (s - 2);
+        for (int q = threadIdx.x; q < S / 4; q += blockDim.x) {
+            const int j = q & (h - 1);
+            const int e0 = ((q >> (s - 2)) << s) + j;
             T sn, cs;
-            twiddle((T)((T)j / (T)half), sn, cs);            // exp(+i*pi*j/half)
-            const V u = x[i0], w = x[i1];
-            const V d = {u.x - w.x, u.y - w.y};
-            x[i0] = V{u.x + w.x, u.y + w.y};
-            x[i1] = V{d.x * cs - d.y * sn, d.x * sn + d.y * cs};
+            twiddle((T)((T)j / (T)(2 * h)), sn, cs);
+            const V c1 = V{cs, sn};                  // exp(+i pi j / 2h)
+            const V c2 = V{-sn, cs};                 // +i c1
+            const V c0 = cmul(c1, c1);               // exp(+i pi j / h)
+            const V x0 = x[e0], x1 = x[e0 + h], x2 = x[e0 + 2 * h], x3 = x[e0 + 3 * h];
+            const V b0 = cadd(x0, x2), b2 = cmul(csub(x0, x2), c1);
+            const V b1 = cadd(x1, x3), b3 = cmul(csub(x1, x3), c2);
+            x[e0] = cadd(b0, b1);
+            x[e0 + h] = cmul(csub(b0, b1), c0);
+            x[e0 + 2 * h] = cadd(b2, b3);
+            x[e0 + 3 * h] = cmul(csub(b2, b3), c0);
+        }
+        __syncthreads();
+    }
+    if (s == 1) {
+        for (int b = threadIdx.x; b < S / 2; b += blockDim.x) {
+            const V u = x[2 * b], v = x[2 * b + 1];
+            x[2 * b] = cadd(u, v);
+            x[2 * b + 1] = csub(u, v);
         }
         __syncthreads();
     }

@@ -118,7 +118,7 @@ def test_sim_then_proc_round_trip(tmp_path, bsar, oracle):
     """bsar_sim writes a workload's file set (GPU forward model); bsar_proc
     images it; the result equals the oracle run on the same files' contents."""
     r = subprocess.run(["python", str(PKG_DIR / "sim.py"), "--workload", "small", "--out", str(tmp_path),
-                        "--pulses", "16", "--grid", "6", "5", "4", "--samples", "512"],
+                        "--pulses", "16", "--grid", "6", "5", "4", "--samples", "1024"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     r = run_tool("--fc", "1e9", "--fs", "800e6", cwd=tmp_path)
