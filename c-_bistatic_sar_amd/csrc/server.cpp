@@ -11,8 +11,9 @@
 //   3. replays the captured graph of the round's buffer [work-list copy from
 //      pinned memory -> one persistent kernel per variant present], then the
 //      final group sums of jobs whose last window is in the round;
-//   4. waits for the previous round, copies finished images to the callers'
-//      buffers and releases their slots and memory.
+//   4. waits for the previous round, copies finished images (already in
+//      pinned memory: their DMA follows the round on its stream) to the
+//      callers' buffers and releases their slots and memory.
 // Rounds alternate between two list buffers, so the host builds round r + 1
 // while the device runs round r.  Windows of one job run in pulse order on
 // one stream, and every voxel of a job sums its pulses in the order of that
@@ -59,6 +60,8 @@ struct Job {
     double r1_exact_frac = 0;
     void* dmem = nullptr;         // accumulators [groups][V] (16 B each) | axes
     size_t dcap = 0;
+    void* stage = nullptr;        // pinned copy of the finished image
+    size_t scap = 0;
     int variant = 0;
     double cost_vp = 0;           // lane instructions per voxel-pulse (estimate)
     int64_t tiles = 0;
@@ -91,6 +94,31 @@ struct Pool {
         }
         *cap = bytes;
         return e;
+    }
+    void put(void* p, size_t cap) { free_.emplace(cap, p); }
+};
+
+// Pinned host blocks reused by size (finished images land here by DMA in
+// stream order; the host then copies them to the callers' buffers).
+struct HostPool {
+    std::multimap<size_t, void*> free_;
+    ~HostPool() { trim(); }
+    void trim()
+    {
+        for (auto& kv : free_) (void)hipHostFree(kv.second);
+        free_.clear();
+    }
+    hipError_t get(size_t bytes, void** p, size_t* cap)
+    {
+        auto it = free_.lower_bound(bytes);
+        if (it != free_.end() && it->first <= 2 * bytes + (1u << 20)) {
+            *p = it->second;
+            *cap = it->first;
+            free_.erase(it);
+            return hipSuccess;
+        }
+        *cap = bytes;
+        return hipHostMalloc(p, bytes, hipHostMallocDefault);
     }
     void put(void* p, size_t cap) { free_.emplace(cap, p); }
 };
@@ -144,6 +172,8 @@ struct bsar_server {
     std::map<int, hipGraphExec_t> graphs[2];   // by variant mask
     int blocks[kSrvVariants] = {0, 0, 0, 0};
     Pool pool;
+    HostPool hpool;
+    unsigned long long* h_counts = nullptr;   // pinned [max_jobs]: exact-R1 counts of finished jobs
     double sec_per_unit = 1.0 / 4.5e13; // calibrated from measured rounds
     std::vector<std::vector<SrvItem>> vitems = std::vector<std::vector<SrvItem>>(kSrvVariants);
 
@@ -229,6 +259,8 @@ void release(bsar_server* s, Job* j)
 {
     if (j->dmem) s->pool.put(j->dmem, j->dcap);
     j->dmem = nullptr;
+    if (j->stage) s->hpool.put(j->stage, j->scap);
+    j->stage = nullptr;
     if (j->slot >= 0) s->free_slots.push_back(j->slot);
     s->active.erase(std::remove(s->active.begin(), s->active.end(), j), s->active.end());
 }
@@ -407,11 +439,17 @@ int launch_round(bsar_server* s, Round& r)
     BSAR_CHECK_HIP(c, get_graph(s, r.buf, mask, &g));
     BSAR_CHECK_HIP(c, hipEventRecord(s->ev_beg[r.buf], s->cs));
     BSAR_CHECK_HIP(c, hipGraphLaunch(g, s->cs));
+    // finished images: group sums, then DMA of the image and its exact-R1
+    // count to pinned memory, in stream order after the round
     for (Job* j : r.finishing) {
         if (j->precision == BSAR_MIXED)
             BSAR_CHECK_HIP(c, finish_mixed((const float4*)j->dmem, j->pl.groups, j->V, (double2*)j->dmem, s->cs));
         else
             BSAR_CHECK_HIP(c, finish_fp64((const double2*)j->dmem, j->pl.groups, j->V, (double2*)j->dmem, s->cs));
+        BSAR_CHECK_HIP(c, s->hpool.get((size_t)j->V * 16, &j->stage, &j->scap));
+        BSAR_CHECK_HIP(c, hipMemcpyAsync(j->stage, j->dmem, (size_t)j->V * 16, hipMemcpyDeviceToHost, s->cs));
+        BSAR_CHECK_HIP(c, hipMemcpyAsync(s->h_counts + j->slot, s->d_counts + j->slot, sizeof(unsigned long long),
+                                         hipMemcpyDeviceToHost, s->cs));
     }
     BSAR_CHECK_HIP(c, hipEventRecord(s->ev_end[r.buf], s->cs));
     return BSAR_OK;
@@ -435,12 +473,10 @@ int complete_round(bsar_server* s, Round& r)
         s->st.device_seconds += sec;
     }
     for (Job* j : r.finishing) {
-        unsigned long long exact = 0;
-        if (j->V > 0)
-            BSAR_CHECK_HIP(c, hipMemcpyAsync(j->out, j->dmem, (size_t)j->V * 16, hipMemcpyDeviceToHost, c->stream));
-        BSAR_CHECK_HIP(c, hipMemcpyAsync(&exact, s->d_counts + j->slot, sizeof exact, hipMemcpyDeviceToHost,
-                                         c->stream));
-        BSAR_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+        const unsigned long long exact = s->h_counts[j->slot];
+        if (j->V > 0) std::memcpy(j->out, j->stage, (size_t)j->V * 16);
+        s->hpool.put(j->stage, j->scap);
+        j->stage = nullptr;
         j->r1_exact_frac = j->precision == BSAR_MIXED ? (double)exact / (2.0 * (double)j->V * j->K) : 1.0;
         release(s, j);
         notify_done(s, j, BSAR_JOB_DONE, BSAR_OK);
@@ -541,7 +577,9 @@ void destroy_server(bsar_server* s)
     if (s->h_axes) (void)hipHostFree(s->h_axes);
     if (s->d_jobs) (void)hipFree(s->d_jobs);
     if (s->d_counts) (void)hipFree(s->d_counts);
+    if (s->h_counts) (void)hipHostFree(s->h_counts);
     s->pool.trim();
+    s->hpool.trim();
     if (s->cs) (void)hipStreamDestroy(s->cs);
     if (s->ctx) bsar_ctx_destroy(s->ctx);
     for (auto& kv : s->jobs) delete kv.second;
@@ -586,6 +624,9 @@ int bsar_server_create(int device, const bsar_server_config* cfg, bsar_server** 
         return fail(e);
     if ((e = hipMalloc((void**)&s->d_jobs, (size_t)M * sizeof(SrvJobDev))) != hipSuccess) return fail(e);
     if ((e = hipMalloc((void**)&s->d_counts, (size_t)M * sizeof(unsigned long long))) != hipSuccess) return fail(e);
+    if ((e = hipHostMalloc((void**)&s->h_counts, (size_t)M * sizeof(unsigned long long), hipHostMallocDefault)) !=
+        hipSuccess)
+        return fail(e);
     if ((e = alloc_lists(s, kSrvListCap)) != hipSuccess) return fail(e);
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
