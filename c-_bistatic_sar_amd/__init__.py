@@ -54,6 +54,10 @@ class ReportC(C.Structure):
                 ("pulses", C.c_int32), ("launches", C.c_int32), ("r1_exact_frac", C.c_double)]
 
 
+class ChirpC(C.Structure):
+    _fields_ = [("f0", C.c_double), ("f1", C.c_double), ("T", C.c_double), ("fs", C.c_double)]
+
+
 class ServerConfigC(C.Structure):
     _fields_ = [("max_jobs", C.c_int), ("reserved0", C.c_int), ("hbm_cap_bytes", C.c_int64),
                 ("round_seconds", C.c_double), ("reserved", C.c_double * 4)]
@@ -91,6 +95,8 @@ SIGNATURES = {
     "bsar_slant_range_fermat": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int64, _dp, _dp]),
     "bsar_slant_range_fermat_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                                  C.c_void_p, C.POINTER(ReportC)]),
+    "bsar_process_raw": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp, C.c_int, C.POINTER(ChirpC), C.c_int, _dp, _dp,
+                                   _dp]),
     "bsar_upsample": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp]),
     "bsar_range_compress": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp]),
     "bsar_range_compress_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_int,
@@ -350,6 +356,18 @@ class Dataset:
         self.ctx._chk(self.L.bsar_slant_range_fermat(self.ctx.h, self.h, _ptr(p), len(p), _ptr(ida), _ptr(volta)),
                       "slant_range_fermat")
         return ida, volta
+
+    def process_raw(self, scat, rcs, chirp: tuple, S: int) -> dict:
+        """processRawData on the GPU: scat [ns][3], rcs [ns] complex, chirp =
+        (f0, f1, T, fs_tx); returns aux, raw, root as [N][S] complex (pulse-major)."""
+        sc = _f64(scat).reshape(-1, 3)
+        r = _c128(np.broadcast_to(np.asarray(rcs, np.complex128), (len(sc),)))
+        ch = ChirpC(*map(float, chirp))
+        out = {k: np.empty((self.n, S), np.complex128) for k in ("aux", "raw", "root")}
+        self.ctx._chk(self.L.bsar_process_raw(self.ctx.h, self.h, _ptr(sc), _ptr(r.view(np.float64)), len(sc),
+                                              C.byref(ch), S, *(_ptr(out[k].view(np.float64))
+                                                                for k in ("aux", "raw", "root"))), "process_raw")
+        return out
 
     def slant_range_fermat_device(self, d_pts: int, V: int, d_ida: int, d_volta: int) -> dict:
         """Device-pointer form (caller-owned buffers: pts [V][3], ida/volta [V][N][3]);

@@ -240,36 +240,13 @@ def reference_signal(chirp: np.ndarray, S: int) -> np.ndarray:
     return ref
 
 
-def place_responses(scene: Scene, ida: np.ndarray, volta: np.ndarray) -> np.ndarray:
-    """auxData as [N][S] (pulse-major): delays and phases of
-    raw_data_processing.cpp:322-361 from refraction points ida/volta
-    [ns][N][3]; responses of several scatterers add."""
-    ns, N = ida.shape[:2]
-    P = scene.scat[:, None, :]
-    T = scene.tx[None, :, :]
-    R = scene.rx[None, :, :]
-    d = lambda a, b: np.sqrt(((a - b) ** 2).sum(-1))
-    r1Tx, r2Tx = d(T, ida), d(P, ida)
-    r1Rx, r2Rx = d(R, volta), d(P, volta)
-    total = scene.n1 * r1Tx + scene.n2 * r2Tx + scene.n1 * r1Rx + scene.n2 * r2Rx
-    t = total / scene.c
-    rbin = 1 + np.round(t * scene.fs).astype(np.int64)
-    lam = scene.c / scene.fc
-    phi = -(2.0 * np.pi / lam) * total
-    resp = scene.rcs[:, None] * np.exp(1j * phi)
-    aux = np.zeros((N, scene.S), np.complex128)
-    ok = (rbin >= 0) & (rbin < scene.S)
-    pulse = np.broadcast_to(np.arange(N)[None, :], rbin.shape)
-    np.add.at(aux, (pulse[ok], rbin[ok]), resp[ok])
-    return aux
-
-
 def simulate(ctx, scene: Scene, precision: int = 0, compress: bool = True) -> np.ndarray:
-    """Range-compressed data [N][S] (range fastest, the layout BP reads) on
-    the GPU: refraction points (Fermat kernel), response placement, chirp
-    convolution, optional complex Gaussian noise at scene.snr_db (power
-    relative to the mean raw-signal power), then pulse compression
-    (compress=False returns the raw data before it)."""
+    """Range-compressed data [N][S] (range fastest, the layout BP reads):
+    processRawData on the GPU (bsar_process_raw: refraction points, response
+    placement, chirp convolution, pulse compression), with optional complex
+    Gaussian noise at scene.snr_db (power relative to the mean raw-signal
+    power) added to the raw data before the compression (compress=False
+    returns the raw data)."""
     import importlib.util
     import sys
     from pathlib import Path
@@ -282,21 +259,23 @@ def simulate(ctx, scene: Scene, precision: int = 0, compress: bool = True) -> np
     params = pkg.Params(scene.fc, scene.c, scene.fs, scene.n1, scene.n2, scene.upsample)
     ds = ctx.dataset(params, pkg.Dem(*scene.dem), scene.tx, scene.rx)
     try:
-        ida, volta = ds.slant_range_fermat(scene.scat)
+        out = ds.process_raw(scene.scat, scene.rcs, (-scene.bw / 2, scene.bw / 2, scene.T, scene.fs), scene.S)
     finally:
         ds.close()
-    aux = place_responses(scene, ida, volta)
-    chirp = pkg.chirp(-scene.bw / 2, scene.bw / 2, scene.T, scene.fs)
-    ref = reference_signal(chirp, scene.S)
-    raw = ctx.range_compress(aux, ref, conj=False, precision=precision)
-    if scene.snr_db is not None:
+    raw = out["raw"]
+    if scene.snr_db is None:
+        if not compress:
+            return raw
+        if precision == 0:
+            return out["root"]
+    else:
         rng = np.random.default_rng(scene.seed + 7)
         p = np.mean(np.abs(raw) ** 2)
         sigma = np.sqrt(p / 10 ** (scene.snr_db / 10) / 2)
         raw = raw + sigma * (rng.normal(size=raw.shape) + 1j * rng.normal(size=raw.shape))
-    if not compress:
-        return raw
-    return ctx.range_compress(raw, ref, conj=True, precision=precision)
+        if not compress:
+            return raw
+    return ctx.range_compress(raw, matched_filter(scene), conj=True, precision=precision)
 
 
 def matched_filter(scene: Scene) -> np.ndarray:

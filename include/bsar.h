@@ -87,6 +87,10 @@ typedef struct {
     double r1_exact_frac;  /* MIXED: fraction of R1 means computed exactly (outside the table) */
 } bsar_report;
 
+/* Transmitted pulse of createCHIRP (raw_data_processing.cpp:257-290):
+ * RadarTx.FreqMenor, FreqMaior, DuracaoPulso, fs (radar_init.cpp:17-27). */
+typedef struct { double f0, f1, T, fs; } bsar_chirp_params;
+
 /* ---- context ------------------------------------------------------------ */
 BSAR_API int  bsar_ctx_create(int device, bsar_ctx** out);
 BSAR_API void bsar_ctx_destroy(bsar_ctx* ctx);
@@ -114,6 +118,22 @@ BSAR_API int bsar_refraction_points(bsar_ctx* ctx, const bsar_dem* dem, const do
  * pts is [V][3]; ida and volta are [V][N][3]. */
 BSAR_API int bsar_slant_range_fermat(bsar_ctx* ctx, const bsar_dataset* ds, const double* pts, int64_t V,
                             double* ida, double* volta);
+
+/* processRawData, raw_data_processing.cpp:292-452 (RawDataResult,
+ * raw_data_processing.h:38-43), on the GPU over the dataset's DEM, antennas
+ * and constants (fc, c, n1, n2; params.fs is RadarRx.fs): for each of ns
+ * point scatterers scat[ns][3] with complex amplitude rcs[ns] and each pulse,
+ * the refraction points and slant ranges of calculaSlantRangeFermat, the
+ * response rcs * exp(-i 2 pi / lambda * L) at row 1 + round(L / c * fs)
+ * (:322-361; rows outside [0, S) dropped; the reference simulates one Target,
+ * several scatterers add in order); raw = fftConvolution of each pulse with
+ * the fftshifted chirp reference (:371-421) and root = fftCorrelation of raw
+ * with it (pulse compression, :431-446), FP64.  S (System.IndiceMaximo) is a
+ * power of two at least the chirp length.  Outputs (any may be NULL) are
+ * [N][S] complex, pulse-major (RawDataResult's column i is row i here: the
+ * range-fastest layout rootData_complex.bin and the back-projection use). */
+BSAR_API int bsar_process_raw(bsar_ctx* ctx, const bsar_dataset* ds, const double* scat, const double* rcs, int ns,
+                     const bsar_chirp_params* chirp, int S, double* aux, double* raw, double* root);
 
 /* Same with caller-owned device buffers (d_pts [V][3], d_ida / d_volta
  * [V][N][3]) for callers that keep the refraction points on the GPU; rep
@@ -148,6 +168,7 @@ BSAR_API int bsar_range_compress_device(bsar_ctx* ctx, void* d_sig, int S, int N
 BSAR_API int bsar_chirp(double f0, double f1, double T, double fs, double* out);
 
 /* ---- datasets (what proc_test_script.cpp:225-292 loads) ----------------- */
+
 
 /* DEM, Tx/Rx positions ([n_pulses][3], Tx_positions.csv / Rx_positions.csv)
  * and constants.  The dataset keeps them resident on the context's GPU. */

@@ -213,3 +213,52 @@ def test_load_rootdata_device_matches_host(bsar, ctx):
                       b.backproject(t.gx, t.gy, t.gz, bsar.MIXED)[0].view(np.float64))
     a.close()
     b.close()
+
+
+def test_process_raw_matches_reference(bsar, ctx, golden):
+    """processRawData of the compiled reference (one target, rcs 1, its radar
+    constants, S = 256): same delay bins, responses within the last bit of
+    cos/sin, raw and rootData within 1e-12 of their peaks."""
+    g = golden
+    fc, c, fs_rx, fs_tx, f0, f1, T, n1, n2 = g["sim_args"]
+    S = int(g["sim_S"][0])
+    ds = ctx.dataset(bsar.Params(fc, c, fs_rx, n1, n2), bsar.Dem(g["dem_bumps_x"], g["dem_bumps_y"], g["dem_bumps_z"]),
+                     g["sr_tx"], g["sr_rx"])
+    out = ds.process_raw(g["sr_P"][None, :], [1.0], (f0, f1, T, fs_tx), S)
+    ds.close()
+    cplx = lambda a: (a[..., 0] + 1j * a[..., 1]).T            # [S][n] -> [n][S]
+    aux, raw, root = cplx(g["sim_aux"]), cplx(g["sim_raw"]), cplx(g["sim_root"])
+    assert np.array_equal(out["aux"] != 0, aux != 0) and np.count_nonzero(aux) == aux.shape[0]
+    assert np.abs(out["aux"] - aux).max() <= 4e-16
+    for k, want in (("raw", raw), ("root", root)):
+        assert np.abs(out[k] - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_process_raw_several_scatterers_vs_oracle(bsar, ctx, oracle):
+    from _scenes import Tiny
+    t = Tiny(n=20, S=1024)
+    rng = np.random.default_rng(17)
+    pts = np.stack([rng.uniform(19.8, 20.3, 5), rng.uniform(-0.2, 0.2, 5), rng.uniform(-5.3, -4.8, 5)], 1)
+    rcs = rng.normal(size=5) + 1j * rng.normal(size=5)
+    sim = dict(fc=t.fc, c=t.c, fs=t.fs, f0=-150e6, f1=150e6, T=1e-7, fs_chirp=t.fs, n1=t.n1, n2=t.n2, S=t.S)
+    aux, raw, root = oracle.simulate(sim, t.dem, t.tx, t.rx, pts, rcs)
+    ds = ctx.dataset(t.params(bsar), bsar.Dem(*t.dem), t.tx, t.rx)
+    out = ds.process_raw(pts, rcs, (-150e6, 150e6, 1e-7, t.fs), t.S)
+    ds.close()
+    for k, want in (("aux", aux), ("raw", raw), ("root", root)):
+        want = np.ascontiguousarray(want.T)
+        assert np.abs(out[k] - want).max() <= 1e-12 * np.abs(want).max(), k
+    assert np.array_equal(out["aux"] != 0, aux.T != 0)
+
+
+def test_scene_forward_model_vs_oracle(bsar, ctx, oracle, scene_mod):
+    """The workloads' data path (scene.simulate over bsar_process_raw): 64
+    scatterers, B = 500 MHz, against the oracle's processRawData restatement."""
+    sc = scene_mod.make_scene("wide", n_pulses=16, samples=2048)
+    root = scene_mod.simulate(ctx, sc)
+    _, raw_o, root_o = oracle.simulate(sc.sim_params(), sc.dem, sc.tx, sc.rx, sc.scat, sc.rcs)
+    want = np.ascontiguousarray(root_o.T)
+    assert root.shape == want.shape
+    assert np.abs(root - want).max() <= 1e-12 * np.abs(want).max()
+    raw = scene_mod.simulate(ctx, sc, compress=False)
+    assert np.abs(raw - raw_o.T).max() <= 1e-12 * np.abs(raw_o).max()
