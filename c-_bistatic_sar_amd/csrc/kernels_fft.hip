@@ -7,11 +7,16 @@
 // (fft_utils.cpp:103-137) is the same without the conjugate.  Here the
 // reference spectrum H is computed once per call and each column is one
 // workgroup: the column is staged in LDS, transformed in place (radix-2 DIT
-// forward from bit-reversed load, DIF inverse to bit-reversed store), so HBM
-// sees one read and one write per sample.  Columns larger than the LDS budget
+// forward from bit-reversed load, DIF inverse to bit-reversed store, three
+// stages per LDS round trip, twiddles from a precomputed table), so HBM sees
+// one read and one write per sample.  Columns larger than the LDS budget
 // run as log2(S) global Stockham passes per direction.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "bsar_fft.h"
 
@@ -33,17 +38,77 @@ __device__ __forceinline__ V cadd(V a, V b) { return V{a.x + b.x, a.y + b.y}; }
 template <typename V>
 __device__ __forceinline__ V csub(V a, V b) { return V{a.x - b.x, a.y - b.y}; }
 
+// Group of 2^K elements e_m = base + m h of K consecutive radix-2 DIT stages
+// s .. s+K-1 (h = 2^(s-1), j = q mod h): stage s+t pairs (e_m, e_m+2^t) with
+// twiddle exp(-i pi (j + (m mod 2^t) h) / (2^t h)) = W^(2^(K-1-t)) *
+// exp(-i pi (m mod 2^t) / 2^t), W = exp(-i pi j / (2^(K-1) h)) = tw[j S / (2^K h)]
+// where tw[k] = exp(-2 pi i k / S).  Inverse (DIF) groups run the stages in
+// the opposite order with conjugate twiddles.
+template <typename T, int K, bool kInv>
+__device__ __forceinline__ void fft_group(typename C2<T>::type* x, int s, int q, int logS,
+                                          const typename C2<T>::type* __restrict__ tw)
+{
+    using V = typename C2<T>::type;
+    constexpr int G = 1 << K;
+    // forward: h = 2^(s-1) is the smallest distance; inverse: s is the top stage, h = 2^(s-K)
+    const int lh = kInv ? (s - K) : (s - 1);
+    const int h = 1 << lh;
+    const int j = q & (h - 1);
+    const int base = ((q >> lh) << (lh + K)) + j;
+    V w = tw[(int64_t)j << (logS - K - lh)];                         // exp(-i pi j / (2^(K-1) h))
+    if (kInv) w.y = -w.y;
+    // powers: wt[t] = W^(2^(K-1-t)), t = K-1 .. 0
+    V wt[K];
+    wt[K - 1] = w;
+#pragma unroll
+    for (int t = K - 2; t >= 0; --t) wt[t] = cmul(wt[t + 1], wt[t + 1]);
+    V v[G];
+#pragma unroll
+    for (int m = 0; m < G; ++m) v[m] = x[base + m * h];
+    const T sgn = kInv ? (T)1 : (T)-1;
+#pragma unroll
+    for (int tt = 0; tt < K; ++tt) {
+        const int t = kInv ? (K - 1 - tt) : tt;                      // DIT: distance 2^t ascending; DIF: descending
+        const int d = 1 << t;
+#pragma unroll
+        for (int m = 0; m < G; ++m) {
+            if (m & d) continue;
+            const int r = m & (d - 1);
+            // exp(sgn i pi r / d), d <= 4: multiples of an eighth turn
+            constexpr T c8 = (T)0.70710678118654752440;
+            V rot;
+            if (r == 0) rot = V{(T)1, (T)0};
+            else if (2 * r == d) rot = V{(T)0, sgn};
+            else if (r == 1) rot = V{c8, sgn * c8};
+            else rot = V{-c8, sgn * c8};                               // r = 3, d = 4
+            const V tw_m = (r == 0) ? wt[t] : cmul(wt[t], rot);
+            if (!kInv) {
+                const V u = v[m], b = cmul(v[m + d], tw_m);
+                v[m] = cadd(u, b);
+                v[m + d] = csub(u, b);
+            } else {
+                const V u = v[m], b = v[m + d];
+                v[m] = cadd(u, b);
+                v[m + d] = cmul(csub(u, b), tw_m);
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < G; ++m) x[base + m * h] = v[m];
+}
+
 // Fused correlation: out[c] = IFFT(FFT(in[c]) * H) for one column c per block.
 // H is the (already conjugated when correlating) reference spectrum, length
-// S, natural order.  The column is loaded bit-reversed, transformed by
-// radix-2 DIT stages taken two at a time (one LDS round trip and one
-// sin/cos per 4 elements: with w' = exp(-i pi j / 2h) the stage-s twiddle is
-// w'^2 and the second stage-(s+1) twiddle -i w'), multiplied by H, inverse
-// transformed by DIF stages two at a time, and stored bit-reversed / S.
+// S, natural order; tw[k] = exp(-2 pi i k / S), k < S / 2.  The column is
+// loaded bit-reversed, transformed by radix-2 DIT stages taken up to three at
+// a time in registers (one LDS round trip per group), multiplied by H,
+// inverse transformed by DIF stages three at a time, and stored
+// bit-reversed / S.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_fft_filter_lds(const typename C2<T>::type* __restrict__ in,
                                                          typename C2<T>::type* __restrict__ out,
-                                                         const typename C2<T>::type* __restrict__ H, int logS)
+                                                         const typename C2<T>::type* __restrict__ H,
+                                                         const typename C2<T>::type* __restrict__ tw, int logS)
 {
     using V = typename C2<T>::type;
     extern __shared__ unsigned char smem[];
@@ -53,68 +118,35 @@ __global__ __launch_bounds__(1024) void k_fft_filter_lds(const typename C2<T>::t
     const V* src = in + col * S;
     for (int k = threadIdx.x; k < S; k += blockDim.x) x[bitrev(k, logS)] = src[k];
     __syncthreads();
-    // forward DIT; an odd stage count starts with the twiddle-free stage 1
+    // forward DIT: groups of 3 stages, the remainder (1 or 2) first
     int s = 1;
-    if (logS & 1) {
-        for (int b = threadIdx.x; b < S / 2; b += blockDim.x) {
-            const V u = x[2 * b], v = x[2 * b + 1];
-            x[2 * b] = cadd(u, v);
-            x[2 * b + 1] = csub(u, v);
-        }
+    const int rem = logS % 3;
+    if (rem == 1) {
+        for (int q = threadIdx.x; q < S / 2; q += blockDim.x) fft_group<T, 1, false>(x, s, q, logS, tw);
         __syncthreads();
-        s = 2;
+        s += 1;
+    } else if (rem == 2) {
+        for (int q = threadIdx.x; q < S / 4; q += blockDim.x) fft_group<T, 2, false>(x, s, q, logS, tw);
+        __syncthreads();
+        s += 2;
     }
-    for (; s < logS; s += 2) {                       // stages s, s + 1; h = 2^(s-1)
-        const int h = 1 << (s - 1);
-        for (int q = threadIdx.x; q < S / 4; q += blockDim.x) {
-            const int j = q & (h - 1);
-            const int e0 = ((q >> (s - 1)) << (s + 1)) + j;
-            T sn, cs;
-            twiddle((T)(-(T)j / (T)(2 * h)), sn, cs);
-            const V w1 = V{cs, sn};                  // exp(-i pi j / 2h)
-            const V w0 = cmul(w1, w1);               // exp(-i pi j / h)
-            const V w2 = V{sn, -cs};                 // -i w1
-            const V x0 = x[e0], x1 = x[e0 + h], x2 = x[e0 + 2 * h], x3 = x[e0 + 3 * h];
-            const V t1 = cmul(x1, w0), t3 = cmul(x3, w0);
-            const V a0 = cadd(x0, t1), a1 = csub(x0, t1), a2 = cadd(x2, t3), a3 = csub(x2, t3);
-            const V u2 = cmul(a2, w1), u3 = cmul(a3, w2);
-            x[e0] = cadd(a0, u2);
-            x[e0 + 2 * h] = csub(a0, u2);
-            x[e0 + h] = cadd(a1, u3);
-            x[e0 + 3 * h] = csub(a1, u3);
-        }
+    for (; s + 2 <= logS; s += 3) {
+        for (int q = threadIdx.x; q < S / 8; q += blockDim.x) fft_group<T, 3, false>(x, s, q, logS, tw);
         __syncthreads();
     }
     for (int k = threadIdx.x; k < S; k += blockDim.x) x[k] = cmul(x[k], H[k]);
     __syncthreads();
-    // inverse DIF, stages taken two at a time from the top; an odd count
-    // ends with the twiddle-free stage 1
-    for (s = logS; s >= 2; s -= 2) {                 // stages s, s - 1; h = 2^(s-2)
-        const int h = 1 << (s - 2);
-        for (int q = threadIdx.x; q < S / 4; q += blockDim.x) {
-            const int j = q & (h - 1);
-            const int e0 = ((q >> (s - 2)) << s) + j;
-            T sn, cs;
-            twiddle((T)((T)j / (T)(2 * h)), sn, cs);
-            const V c1 = V{cs, sn};                  // exp(+i pi j / 2h)
-            const V c2 = V{-sn, cs};                 // +i c1
-            const V c0 = cmul(c1, c1);               // exp(+i pi j / h)
-            const V x0 = x[e0], x1 = x[e0 + h], x2 = x[e0 + 2 * h], x3 = x[e0 + 3 * h];
-            const V b0 = cadd(x0, x2), b2 = cmul(csub(x0, x2), c1);
-            const V b1 = cadd(x1, x3), b3 = cmul(csub(x1, x3), c2);
-            x[e0] = cadd(b0, b1);
-            x[e0 + h] = cmul(csub(b0, b1), c0);
-            x[e0 + 2 * h] = cadd(b2, b3);
-            x[e0 + 3 * h] = cmul(csub(b2, b3), c0);
-        }
+    // inverse DIF: groups of 3 stages from the top (s = top stage), the
+    // remainder (stages rem .. 1) last
+    for (s = logS; s >= rem + 3; s -= 3) {
+        for (int q = threadIdx.x; q < S / 8; q += blockDim.x) fft_group<T, 3, true>(x, s, q, logS, tw);
         __syncthreads();
     }
-    if (s == 1) {
-        for (int b = threadIdx.x; b < S / 2; b += blockDim.x) {
-            const V u = x[2 * b], v = x[2 * b + 1];
-            x[2 * b] = cadd(u, v);
-            x[2 * b + 1] = csub(u, v);
-        }
+    if (rem == 1) {
+        for (int q = threadIdx.x; q < S / 2; q += blockDim.x) fft_group<T, 1, true>(x, s, q, logS, tw);
+        __syncthreads();
+    } else if (rem == 2) {
+        for (int q = threadIdx.x; q < S / 4; q += blockDim.x) fft_group<T, 2, true>(x, s, q, logS, tw);
         __syncthreads();
     }
     const T inv = (T)1 / (T)S;
@@ -123,6 +155,17 @@ __global__ __launch_bounds__(1024) void k_fft_filter_lds(const typename C2<T>::t
         const V a = x[bitrev(k, logS)];
         dst[k] = V{a.x * inv, a.y * inv};
     }
+}
+
+// tw[k] = exp(-2 pi i k / S), k < S / 2, from FP64 sincospi.
+template <typename T>
+__global__ void k_twiddles(typename C2<T>::type* tw, int S)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S / 2) return;
+    double sn, cs;
+    sincospi(-2.0 * (double)k / (double)S, &sn, &cs);
+    tw[k] = typename C2<T>::type{(T)cs, (T)sn};
 }
 
 // One radix-2 Stockham pass over all columns (global memory): stage with
@@ -222,6 +265,33 @@ static hipError_t fft_global(typename C2<T>::type* a, typename C2<T>::type* tmp,
 template <typename T>
 static size_t lds_bytes(int S) { return (size_t)S * sizeof(typename C2<T>::type); }
 
+// Twiddle tables tw[k] = exp(-2 pi i k / S), k < S / 2, one per (device, S,
+// precision), built on first use and kept for the life of the process.
+template <typename T>
+static hipError_t twiddle_table(int S, hipStream_t st, const typename C2<T>::type** out)
+{
+    using V = typename C2<T>::type;
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, V*> tables;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = tables.find({dev, S});
+    if (it != tables.end()) { *out = it->second; return hipSuccess; }
+    V* tw = nullptr;
+    const int n = S / 2 > 0 ? S / 2 : 1;
+    if ((e = hipMalloc((void**)&tw, (size_t)n * sizeof(V))) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_twiddles<T>, dim3(nblk(n, 256)), dim3(256), 0, st, tw, S);
+    if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess) {
+        (void)hipFree(tw);
+        return e;
+    }
+    tables[{dev, S}] = tw;
+    *out = tw;
+    return hipSuccess;
+}
+
 template <typename T>
 hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, const typename C2<T>::type* H, int S,
                         int64_t ncols, hipStream_t st)
@@ -234,8 +304,12 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
         hipError_t e = hipFuncSetAttribute((const void*)k_fft_filter_lds<T>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
         if (e != hipSuccess) return e;
-        const int threads = S / 2 >= 1024 ? 1024 : (S / 2 < 64 ? 64 : S / 2);
-        hipLaunchKernelGGL(k_fft_filter_lds<T>, dim3((unsigned)ncols), dim3(threads), lb, st, data, data, H, logS);
+        const V* tw = nullptr;
+        if ((e = twiddle_table<T>(S, st, &tw)) != hipSuccess) return e;
+        // one thread per 8-point group of the radix-8 passes
+        const int threads = S / 8 >= 1024 ? 1024 : (S / 8 < 64 ? 64 : S / 8);
+        hipLaunchKernelGGL(k_fft_filter_lds<T>, dim3((unsigned)ncols), dim3(threads), lb, st, data, data, H, tw,
+                           logS);
         return hipGetLastError();
     }
     V* res = nullptr;
