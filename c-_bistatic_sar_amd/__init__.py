@@ -44,6 +44,11 @@ class DemC(C.Structure):
                 ("zrows", C.c_int), ("zcols", C.c_int)]
 
 
+class DemGridC(C.Structure):
+    _fields_ = [("dem", DemC), ("lon0", C.c_double), ("lat0", C.c_double), ("e0", C.c_double), ("n0", C.c_double),
+                ("utm_zone", C.c_int), ("nodata_count", C.c_int), ("source", C.c_int), ("reserved", C.c_int)]
+
+
 class ParamsC(C.Structure):
     _fields_ = [("fc", C.c_double), ("c", C.c_double), ("fs", C.c_double), ("n1", C.c_double),
                 ("n2", C.c_double), ("upsample", C.c_int), ("reserved", C.c_int)]
@@ -119,6 +124,13 @@ SIGNATURES = {
     "bsar_write_image": (C.c_int, [C.c_char_p, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
     "bsar_write_matrix2d": (C.c_int, [C.c_char_p, _dp, C.c_uint64, C.c_uint64]),
     "bsar_flops_per_voxel_pulse": (C.c_double, [C.c_int]),
+    "bsar_file_error": (C.c_char_p, []),
+    "bsar_spiral": (C.c_int, [C.c_double] * 7 + [C.c_int, _dp, _dp, _dp, C.c_int64, C.POINTER(C.c_int64)]),
+    "bsar_grid_axis": (C.c_int, [C.c_double, C.c_double, C.c_double, _dp, C.c_int, C.POINTER(C.c_int)]),
+    "bsar_dem_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(DemGridC)]),
+    "bsar_dem_grid_free": (None, [C.POINTER(DemGridC)]),
+    "bsar_geo_to_utm": (C.c_int, [C.c_double, C.c_double, C.c_int, _dp, _dp, C.POINTER(C.c_int)]),
+    "bsar_meridian_arc": (C.c_double, [C.c_double]),
     "bsar_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfigC), C.POINTER(C.c_void_p)]),
     "bsar_server_destroy": (None, [C.c_void_p]),
     "bsar_server_last_error": (C.c_char_p, [C.c_void_p]),
@@ -434,6 +446,69 @@ def write_matrix2d(path, data):
     st = L.bsar_write_matrix2d(str(path).encode(), _ptr(d.view(np.float64)), rows, cols)
     if st != 0:
         raise BsarError(st, f"write_matrix2d({path})")
+
+
+def _file_check(st: int, what: str):
+    if st != 0:
+        raise BsarError(st, f"{what}: {lib().bsar_file_error().decode()}")
+
+
+def spiral(turns: float, r_minor: float, r_major: float, h_high: float, h_low: float, vt: float, prf: float,
+           mirror_xy: bool = False) -> np.ndarray:
+    """funcao_espiral (funcao_espiral.cpp:5-49) as [n][3] positions; mirror_xy
+    gives generateReceiverTrajectory's (-x, -y) (trajectory_generator.cpp:17-33)."""
+    L = lib()
+    n = C.c_int64()
+    args = (turns, r_minor, r_major, h_high, h_low, vt, prf, int(bool(mirror_xy)))
+    _file_check(L.bsar_spiral(*args, None, None, None, 0, C.byref(n)), "spiral")
+    px, py, pz = (np.empty(n.value) for _ in range(3))
+    _file_check(L.bsar_spiral(*args, _ptr(px), _ptr(py), _ptr(pz), n.value, C.byref(n)), "spiral")
+    return np.stack([px, py, pz], 1)
+
+
+def grid_axis(center: float, step: float, half: float) -> np.ndarray:
+    """One axis of crearGrid (crear_grid.cpp:21-34)."""
+    L = lib()
+    n = C.c_int()
+    _file_check(L.bsar_grid_axis(center, step, half, None, 0, C.byref(n)), "grid_axis")
+    out = np.empty(n.value)
+    _file_check(L.bsar_grid_axis(center, step, half, _ptr(out), n.value, C.byref(n)), "grid_axis")
+    return out
+
+
+DEM_FILE_REFERENCE, DEM_FILE_GEOTIFF = 0, 1
+DEM_SOURCES = {0: "default", 1: "ascii_grid", 2: "geotiff"}
+
+
+def load_dem(path, mode: int = DEM_FILE_REFERENCE) -> tuple[Dem, dict]:
+    """loadDEM (load_dem_simple.cpp:9-224) with mode DEM_FILE_REFERENCE, or
+    a GeoTIFF decoded to local metres with DEM_FILE_GEOTIFF.  Returns the DEM
+    and its metadata (source, centre, UTM zone, nodata count)."""
+    L = lib()
+    g = DemGridC()
+    _file_check(L.bsar_dem_load(str(path).encode(), mode, C.byref(g)), f"load_dem({path})")
+    try:
+        d = g.dem
+        x = np.ctypeslib.as_array(d.x, (d.nx,)).copy() if d.nx else np.empty(0)
+        y = np.ctypeslib.as_array(d.y, (d.ny,)).copy() if d.ny else np.empty(0)
+        n = d.zrows * d.zcols
+        z = np.ctypeslib.as_array(d.z, (n,)).copy().reshape(d.zrows, d.zcols) if n else np.empty((d.zrows, d.zcols))
+        meta = {"source": DEM_SOURCES.get(g.source, g.source), "lon0": g.lon0, "lat0": g.lat0, "e0": g.e0,
+                "n0": g.n0, "utm_zone": g.utm_zone, "nodata_count": g.nodata_count}
+    finally:
+        L.bsar_dem_grid_free(C.byref(g))
+    return Dem(x, y, z), meta
+
+
+def geo_to_utm(lat: float, lon: float, zone: int = 0) -> tuple[float, float, int]:
+    """WGS84 (lat, lon) degrees -> UTM (easting, northing, zone; zone < 0 south)."""
+    e, n, z = C.c_double(), C.c_double(), C.c_int()
+    _file_check(lib().bsar_geo_to_utm(lat, lon, zone, C.byref(e), C.byref(n), C.byref(z)), "geo_to_utm")
+    return e.value, n.value, z.value
+
+
+def meridian_arc(lat: float) -> float:
+    return float(lib().bsar_meridian_arc(lat))
 
 
 def flops_per_voxel_pulse(n: int) -> float:

@@ -127,6 +127,49 @@ def _scatterers(rng, count, box=BOX, margin=0.1):
     return rng.uniform(lo, hi, size=(count, 3))
 
 
+def _pkg():
+    import importlib.util
+    import sys
+    from pathlib import Path
+    pkg = sys.modules.get("bsar_amd")
+    if pkg is None:
+        spec = importlib.util.spec_from_file_location("bsar_amd", Path(__file__).parent / "__init__.py")
+        pkg = importlib.util.module_from_spec(spec)
+        sys.modules["bsar_amd"] = pkg
+        spec.loader.exec_module(pkg)
+    return pkg
+
+
+# The reference's default scenario, gs_test_script.cpp:15-137 with the
+# constants of radar_init.cpp:5-89: carrier 10 GHz, ADC 60 MHz, chirp -25..25
+# MHz over 1.3 us, S = IndiceMaximo = 256 samples; Tx/Rx spirals of 2 turns
+# from 147.5 m to 172.5 m radius, 120 m down to 80 m, 120 m/s, PRF 400
+# (6689 pulses; Rx mirrored); one target at (20, 0, -5) with rcs 1;
+# crearGrid(0.04, 0.8, 0.08, 0.8) around it; loadDEM of the .tif name (the
+# flat 32 x 32 DEM of the build without GDAL); n1 = 1, n2 = 4 (the
+# environment defaults, raw_data_processing.cpp:17-19).
+REFERENCE_DEM = "assets/DEM_1x1km_Res30m_Lat-3_6160_Lon-80_4552.tif"
+REFERENCE_SPIRAL = (2, 172.5, 147.5, 120.0, 80.0, 120.0, 400.0)
+
+
+def reference_scene(n_pulses: int | None = None, dem_path: str | None = None, dem_mode: int = 0) -> Scene:
+    """The reference scenario; dem_path/dem_mode replace the DEM by a file
+    read with load_dem (mode 1 decodes a GeoTIFF)."""
+    pkg = _pkg()
+    tx = pkg.spiral(*REFERENCE_SPIRAL)
+    rx = pkg.spiral(*REFERENCE_SPIRAL, mirror_xy=True)
+    if n_pulses is not None:
+        tx, rx = tx[:n_pulses], rx[:n_pulses]
+    dem, meta = pkg.load_dem(dem_path or REFERENCE_DEM, dem_mode)
+    target = np.array([20.0, 0.0, -5.0])
+    gx = pkg.grid_axis(target[0], 0.04, 0.8)
+    gy = pkg.grid_axis(target[1], 0.04, 0.8)
+    gz = pkg.grid_axis(target[2], 0.08, 0.8)
+    return Scene(name="reference", fc=10e9, c=3e8, fs=60e6, bw=50e6, T=1.3e-6, n1=1.0, n2=4.0, S=256, tx=tx, rx=rx,
+                 dem=(dem.x, dem.y, dem.z), gx=gx, gy=gy, gz=gz, scat=target[None, :], rcs=np.ones(1, np.complex128),
+                 extra={"target": target, "dem_meta": meta})
+
+
 def make_scene(name: str, seed: int | None = None, n_pulses: int | None = None, grid=None, samples=None,
                clutter: int | None = None) -> Scene:
     """Workloads:
@@ -137,7 +180,12 @@ def make_scene(name: str, seed: int | None = None, n_pulses: int | None = None, 
       serve   resident 4096-pulse x 8192-sample dataset for the serving stream
               (serve_jobs: grids and pulse counts of the jobs), 16 scatterers
     Every one: fc = 1 GHz, T = 1 us, fs = 800 MHz, c = 3e8, n1 = 1, n2 = sqrt(eps_r).
+      reference  the reference's own default scenario (reference_scene)
     """
+    if name == "reference":
+        if grid is not None or samples is not None:
+            raise ValueError("the reference scenario fixes its grid and samples")
+        return reference_scene(n_pulses)
     seeds = dict(small=1001, large=1002, rough=1003, wide=1004, serve=1005)
     seed = seeds.get(name, 1000) if seed is None else seed
     rng = np.random.default_rng(seed)
@@ -240,65 +288,58 @@ def reference_signal(chirp: np.ndarray, S: int) -> np.ndarray:
     return ref
 
 
-def simulate(ctx, scene: Scene, precision: int = 0, compress: bool = True) -> np.ndarray:
-    """Range-compressed data [N][S] (range fastest, the layout BP reads):
-    processRawData on the GPU (bsar_process_raw: refraction points, response
-    placement, chirp convolution, pulse compression), with optional complex
-    Gaussian noise at scene.snr_db (power relative to the mean raw-signal
-    power) added to the raw data before the compression (compress=False
-    returns the raw data)."""
-    import importlib.util
-    import sys
-    from pathlib import Path
-    pkg = sys.modules.get("bsar_amd")
-    if pkg is None:
-        spec = importlib.util.spec_from_file_location("bsar_amd", Path(__file__).parent / "__init__.py")
-        pkg = importlib.util.module_from_spec(spec)
-        sys.modules["bsar_amd"] = pkg
-        spec.loader.exec_module(pkg)
+def forward_model(ctx, scene: Scene, precision: int = 0) -> dict:
+    """processRawData's three outputs for the scene, [N][S] complex each
+    (range fastest, the layout BP reads): aux (placed responses), raw
+    (convolved with the chirp) and root (pulse-compressed), from the GPU
+    (bsar_process_raw: refraction points, response placement, chirp
+    convolution, pulse compression).  With scene.snr_db, complex Gaussian
+    noise (power relative to the mean raw-signal power) is added to raw
+    before the compression."""
+    pkg = _pkg()
     params = pkg.Params(scene.fc, scene.c, scene.fs, scene.n1, scene.n2, scene.upsample)
     ds = ctx.dataset(params, pkg.Dem(*scene.dem), scene.tx, scene.rx)
     try:
         out = ds.process_raw(scene.scat, scene.rcs, (-scene.bw / 2, scene.bw / 2, scene.T, scene.fs), scene.S)
     finally:
         ds.close()
-    raw = out["raw"]
-    if scene.snr_db is None:
-        if not compress:
-            return raw
-        if precision == 0:
-            return out["root"]
-    else:
+    if scene.snr_db is not None:
+        raw = out["raw"]
         rng = np.random.default_rng(scene.seed + 7)
         p = np.mean(np.abs(raw) ** 2)
         sigma = np.sqrt(p / 10 ** (scene.snr_db / 10) / 2)
-        raw = raw + sigma * (rng.normal(size=raw.shape) + 1j * rng.normal(size=raw.shape))
-        if not compress:
-            return raw
-    return ctx.range_compress(raw, matched_filter(scene), conj=True, precision=precision)
+        out["raw"] = raw + sigma * (rng.normal(size=raw.shape) + 1j * rng.normal(size=raw.shape))
+    if scene.snr_db is not None or precision != 0:
+        out["root"] = ctx.range_compress(out["raw"], matched_filter(scene), conj=True, precision=precision)
+    return out
+
+
+def simulate(ctx, scene: Scene, precision: int = 0, compress: bool = True) -> np.ndarray:
+    """Range-compressed data [N][S] of forward_model (compress=False returns
+    the raw data)."""
+    out = forward_model(ctx, scene, precision)
+    return out["root"] if compress else out["raw"]
 
 
 def matched_filter(scene: Scene) -> np.ndarray:
     """The pulse-compression reference of processRawData for this scene."""
-    import importlib.util
-    import sys
-    from pathlib import Path
-    pkg = sys.modules.get("bsar_amd")
-    if pkg is None:
-        spec = importlib.util.spec_from_file_location("bsar_amd", Path(__file__).parent / "__init__.py")
-        pkg = importlib.util.module_from_spec(spec)
-        sys.modules["bsar_amd"] = pkg
-        spec.loader.exec_module(pkg)
+    pkg = _pkg()
     return reference_signal(pkg.chirp(-scene.bw / 2, scene.bw / 2, scene.T, scene.fs), scene.S)
 
 
-def write_reference_inputs(proc, sc: Scene, root: np.ndarray, gx=None, gy=None, gz=None) -> None:
-    """The build/output/proc/ file set that proc_test_script.cpp reads
-    (:225-292), in the reference's formats: rootData_complex.bin as
-    data_export.cpp:71-138 writes a 2-D complex matrix (header + payload, the
-    payload range-fastest: pulse p's samples at r + p*S, proc_test_script.cpp
-    :330), CSVs with six decimals (data_export.cpp:148,169), environment.txt
-    as raw_data_processing.cpp:14-46 parses it."""
+def write_reference_inputs(proc, sc: Scene, root: np.ndarray, gx=None, gy=None, gz=None, raw=None,
+                           aux=None) -> None:
+    """The build/output/proc/ file set of exportProcessedData
+    (data_export.cpp:16-69), which proc_test_script.cpp reads (:225-292), in
+    the reference's formats: rootData_complex.bin (and rawData / auxData when
+    given) as data_export.cpp:71-138 writes a 2-D complex matrix, header dims
+    (S, N), with the payload range-fastest -- pulse p's samples at r + p*S,
+    the order proc_test_script.cpp:330 reads (the reference's writer emits
+    r*N + p, which its reader does not undo); CSVs with six decimals
+    (data_export.cpp:148,169); target_position.csv is the scene's target (the
+    first scatterer); environment.txt as raw_data_processing.cpp:14-46 parses
+    it, each value in the shortest "%.17g" text (the reference's default
+    stream precision, 6 digits, agrees whenever that is exact)."""
     import struct
     from pathlib import Path
     proc = Path(proc)
@@ -307,9 +348,12 @@ def write_reference_inputs(proc, sc: Scene, root: np.ndarray, gx=None, gy=None, 
     gy = sc.gy if gy is None else gy
     gz = sc.gz if gz is None else gz
     N, S = root.shape
-    with open(proc / "rootData_complex.bin", "wb") as f:
-        f.write(struct.pack("<III", 2, 1, 6) + b"double".ljust(16, b"\0") + struct.pack("<QQ", S, N))
-        f.write(np.ascontiguousarray(root, dtype=np.complex128).tobytes())
+    for name, data in (("rootData_complex.bin", root), ("rawData_complex.bin", raw), ("auxData_complex.bin", aux)):
+        if data is None:
+            continue
+        with open(proc / name, "wb") as f:
+            f.write(struct.pack("<III", 2, 1, 6) + b"double".ljust(16, b"\0") + struct.pack("<QQ", S, N))
+            f.write(np.ascontiguousarray(data, dtype=np.complex128).tobytes())
 
     def vec(name, v):
         (proc / name).write_text(",".join("%.6f" % x for x in v) + "\n")
@@ -325,4 +369,14 @@ def write_reference_inputs(proc, sc: Scene, root: np.ndarray, gx=None, gy=None, 
     mat("DEM_Z_matrix.csv", sc.dem[2])
     mat("Tx_positions.csv", sc.tx)
     mat("Rx_positions.csv", sc.rx)
-    (proc / "environment.txt").write_text("n1=%r\nn2=%r\n" % (sc.n1, sc.n2))
+    vec("target_position.csv", sc.extra.get("target", sc.scat[0]))
+    (proc / "environment.txt").write_text("n1=%s\nn2=%s\n" % (_g17(sc.n1), _g17(sc.n2)))
+
+
+def _g17(v: float) -> str:
+    """Shortest of %.6g / %.17g text that reads back as v."""
+    for fmt in ("%.6g", "%.17g"):
+        t = fmt % v
+        if float(t) == v:
+            return t
+    return repr(v)

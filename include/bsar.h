@@ -229,6 +229,61 @@ BSAR_API int bsar_write_image(const char* path, const double* img, int nx, int n
  * dims (rows, cols), data row-major. */
 BSAR_API int bsar_write_matrix2d(const char* path, const double* data, uint64_t rows, uint64_t cols);
 
+/* ---- scenario generators and DEM files (host only, no context) ----------
+ * Context-free calls; on failure bsar_file_error() holds the calling
+ * thread's last message. */
+
+BSAR_API const char* bsar_file_error(void);
+
+/* funcao_espiral, funcao_espiral.cpp:5-49: spiral from radius r_major to
+ * r_minor (the reference passes RaioMenorEsp second, RaioMaiorEsp third,
+ * trajectory_generator.cpp:6-12) over `turns` turns, height h_high -> h_low,
+ * tangential speed vt, one position per 1/prf s while t <= T.  mirror_xy
+ * negates x and y: generateReceiverTrajectory, trajectory_generator.cpp:17-33.
+ * With px/py/pz NULL only *count is returned.  BSAR_ERR_INVALID when the
+ * reference's time loop would not end (prf <= 0, zero radial rate with
+ * distinct radii) or cap is too small. */
+BSAR_API int bsar_spiral(double turns, double r_minor, double r_major, double h_high, double h_low, double vt,
+                         double prf, int mirror_xy, double* px, double* py, double* pz, int64_t cap,
+                         int64_t* count);
+
+/* One axis of crearGrid, crear_grid.cpp:21-34: center-half, +step, ... while
+ * <= center+half+step/2 (a running sum, as the reference builds it).  With
+ * out NULL only *count is returned. */
+BSAR_API int bsar_grid_axis(double center, double step, double half, double* out, int cap, int* count);
+
+/* A DEM read from a file.  dem's arrays are allocated by the library and
+ * released by bsar_dem_grid_free. */
+typedef struct {
+    bsar_dem dem;
+    double lon0, lat0;   /* geographic GeoTIFF: raster centre [deg] (NaN otherwise) */
+    double e0, n0;       /* GeoTIFF: the centre (the origin of x, y) as UTM easting/northing
+                            (geographic) or model metres (projected) */
+    int utm_zone;        /* geographic GeoTIFF: UTM zone of the centre, negative south */
+    int nodata_count;    /* GeoTIFF: nodata pixels, replaced by the mean of the valid ones */
+    int source;          /* BSAR_DEM_SOURCE_* */
+    int reserved;
+} bsar_dem_grid;
+
+enum {
+    BSAR_DEM_FILE_REFERENCE = 0, /* loadDEM without GDAL, load_dem_simple.cpp:9-224: a name containing
+                                    ".tif"/".TIF" or a file that cannot be opened gives the flat 32 x 32
+                                    DEM over [-500, 500] m; anything else is read as an ESRI ASCII grid */
+    BSAR_DEM_FILE_GEOTIFF = 1    /* decode a single-band GeoTIFF to local metres about its centre */
+};
+enum { BSAR_DEM_SOURCE_DEFAULT = 0, BSAR_DEM_SOURCE_ASCII_GRID = 1, BSAR_DEM_SOURCE_GEOTIFF = 2 };
+
+BSAR_API int bsar_dem_load(const char* path, int mode, bsar_dem_grid* out);
+BSAR_API void bsar_dem_grid_free(bsar_dem_grid* g);
+
+/* Geodesy of the GeoTIFF reader (WGS84; no counterpart in the reference,
+ * whose GDAL loader documents the conversion, load_dem.h:9-17): UTM
+ * easting/northing of (lat, lon) in degrees in `zone` (0: the zone of lon),
+ * Krueger's series to fourth order in n; *zone_out is negative south of the
+ * equator.  bsar_meridian_arc: metres from the equator to lat. */
+BSAR_API int bsar_geo_to_utm(double lat, double lon, int zone, double* easting, double* northing, int* zone_out);
+BSAR_API double bsar_meridian_arc(double lat);
+
 /* ---- accounting --------------------------------------------------------- */
 
 /* Algorithmic FP64 flop per voxel-pulse of the plain form: two 709-candidate

@@ -337,5 +337,153 @@ std::vector<std::vector<std::vector<std::complex<double>>>> backProjection(
     return out;
 }
 
+// ---- scenario generators and DEM files (host only, no GPU context) --------
+// The results convert to the caller's own structs (any type with the
+// reference's member names), so `TrajectoryPoints t = funcao_espiral(...)`
+// and `Grid3D g = crearGrid(...)` compile unchanged.
+
+inline void check_file(int st, const char* what)
+{
+    if (st != BSAR_OK)
+        throw std::runtime_error(std::string(what) + ": " + bsar_status_string(st) + " (" + bsar_file_error() + ")");
+}
+
+// TrajectoryPoints, funcao_espiral.h:7-11
+struct Points {
+    std::vector<double> Px, Py, Pz;
+    template <class T>
+    operator T() const
+    {
+        T t;
+        t.Px = Px;
+        t.Py = Py;
+        t.Pz = Pz;
+        return t;
+    }
+};
+
+inline Points spiral_points(double N, double R_minor, double R_major, double H_high, double H_low, double Vt,
+                            double PRF, bool mirror)
+{
+    int64_t n = 0;
+    check_file(bsar_spiral(N, R_minor, R_major, H_high, H_low, Vt, PRF, mirror, nullptr, nullptr, nullptr, 0, &n),
+               "funcao_espiral");
+    Points p;
+    p.Px.resize((size_t)n);
+    p.Py.resize((size_t)n);
+    p.Pz.resize((size_t)n);
+    check_file(bsar_spiral(N, R_minor, R_major, H_high, H_low, Vt, PRF, mirror, p.Px.data(), p.Py.data(),
+                           p.Pz.data(), n, &n), "funcao_espiral");
+    return p;
+}
+
+// funcao_espiral, funcao_espiral.cpp:5-49
+inline Points funcao_espiral(double N, double R_minor, double R_major, double H_high, double H_low, double Vt,
+                             double PRF)
+{
+    return spiral_points(N, R_minor, R_major, H_high, H_low, Vt, PRF, false);
+}
+
+// generateTransmitterTrajectory, trajectory_generator.cpp:4-15 (RADAR:
+// NumVoltasEsp, RaioMenorEsp, RaioMaiorEsp, AltMaiorEsp, AltMenorEsp, Vt, PRF)
+template <class RADAR>
+Points generateTransmitterTrajectory(const RADAR& tx)
+{
+    return spiral_points(tx.NumVoltasEsp, tx.RaioMenorEsp, tx.RaioMaiorEsp, tx.AltMaiorEsp, tx.AltMenorEsp, tx.Vt,
+                         tx.PRF, false);
+}
+
+// generateReceiverTrajectory, trajectory_generator.cpp:17-33: the same spiral with x, y negated
+template <class RADAR>
+Points generateReceiverTrajectory(const RADAR& rx)
+{
+    return spiral_points(rx.NumVoltasEsp, rx.RaioMenorEsp, rx.RaioMaiorEsp, rx.AltMaiorEsp, rx.AltMenorEsp, rx.Vt,
+                         rx.PRF, true);
+}
+
+// Grid3D, crear_grid.h:8-14: axes plus the [ny][nx][nz] meshgrids
+struct GridAxes {
+    std::vector<double> xAxis, yAxis, zAxis;
+    template <class G>
+    operator G() const
+    {
+        G g;
+        g.gridToProc.xAxis = xAxis;
+        g.gridToProc.yAxis = yAxis;
+        g.gridToProc.zAxis = zAxis;
+        g.nx = (int)xAxis.size();
+        g.ny = (int)yAxis.size();
+        g.nz = (int)zAxis.size();
+        const std::vector<double> col(zAxis.size());
+        g.X.assign(yAxis.size(), std::vector<std::vector<double>>(xAxis.size(), col));
+        g.Y = g.X;
+        g.Z = g.X;
+        for (size_t j = 0; j < yAxis.size(); ++j)
+            for (size_t i = 0; i < xAxis.size(); ++i)
+                for (size_t k = 0; k < zAxis.size(); ++k) {
+                    g.X[j][i][k] = xAxis[i];
+                    g.Y[j][i][k] = yAxis[j];
+                    g.Z[j][i][k] = zAxis[k];
+                }
+        return g;
+    }
+};
+
+inline std::vector<double> grid_axis(double center, double step, double half)
+{
+    int n = 0;
+    check_file(bsar_grid_axis(center, step, half, nullptr, 0, &n), "crearGrid");
+    std::vector<double> v((size_t)n);
+    check_file(bsar_grid_axis(center, step, half, v.data(), n, &n), "crearGrid");
+    return v;
+}
+
+// crearGrid, crear_grid.cpp:6-73 (TARGET: pos[3])
+template <class TARGET>
+GridAxes crearGrid(const TARGET& target, double dxy = 0.04, double lxy = 0.8, double dz = 0.08, double lz = 0.8)
+{
+    GridAxes g;
+    g.xAxis = grid_axis(target.pos[0], dxy, lxy);
+    g.yAxis = grid_axis(target.pos[1], dxy, lxy);
+    g.zAxis = grid_axis(target.pos[2], dz, lz);
+    return g;
+}
+
+// ProcessedDEM as loadDEM returns it, radar_params.h:75-79
+struct DemFile {
+    std::vector<double> X_vec, Y_vec;
+    std::vector<std::vector<double>> Z_DEM;
+    template <class D>
+    operator D() const
+    {
+        D d;
+        d.X_vec = X_vec;
+        d.Y_vec = Y_vec;
+        d.Z_DEM = Z_DEM;
+        return d;
+    }
+};
+
+inline DemFile load_dem_file(const std::string& filename, int mode)
+{
+    bsar_dem_grid g;
+    check_file(bsar_dem_load(filename.c_str(), mode, &g), "loadDEM");
+    DemFile d;
+    d.X_vec.assign(g.dem.x, g.dem.x + g.dem.nx);
+    d.Y_vec.assign(g.dem.y, g.dem.y + g.dem.ny);
+    d.Z_DEM.resize((size_t)g.dem.zrows);
+    for (int j = 0; j < g.dem.zrows; ++j)
+        d.Z_DEM[j].assign(g.dem.z + (size_t)j * g.dem.zcols, g.dem.z + (size_t)(j + 1) * g.dem.zcols);
+    bsar_dem_grid_free(&g);
+    return d;
+}
+
+// loadDEM of the build without GDAL, load_dem_simple.cpp:9-224
+inline DemFile loadDEM(const std::string& filename) { return load_dem_file(filename, BSAR_DEM_FILE_REFERENCE); }
+
+// A single-band GeoTIFF decoded to local metres about its centre (what the
+// GDAL loader documents, load_dem.h:9-17)
+inline DemFile loadDEMGeoTIFF(const std::string& filename) { return load_dem_file(filename, BSAR_DEM_FILE_GEOTIFF); }
+
 }  // namespace facade
 }  // namespace bsar

@@ -16,6 +16,7 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
 REF_LIB = HERE / "_ref" / "libref.so"
 REF_PROC = HERE / "_ref" / "proc_test_script"
+REF_GS = HERE / "_ref" / "gs_test_script"
 REF_TREE = Path("/root/reference")
 
 _dp = C.POINTER(C.c_double)
@@ -274,5 +275,126 @@ def ref_lib():
         L.ref_chirp.argtypes = [C.c_double] * 4 + [_dp]
         L.ref_process_raw.argtypes = dem_args + [_dp, _dp, C.c_int, _dp, C.c_double] + [C.c_double] * 9 + \
             [C.c_int, _dp, _dp, _dp]
+        L.ref_spiral.restype = C.c_int64
+        L.ref_spiral.argtypes = [C.c_double] * 7 + [_dp, _dp, _dp, C.c_int64]
+        L.ref_default_trajectory.restype = C.c_int64
+        L.ref_default_trajectory.argtypes = [C.c_int, _dp, C.c_int64]
+        L.ref_grid.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, C.c_int,
+                               C.POINTER(C.c_int)]
+        L.ref_load_dem.argtypes = [C.c_char_p, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.POINTER(C.c_int)]
         _r = L
     return _r
+
+
+def ref_spiral(*args) -> np.ndarray:
+    """funcao_espiral of the compiled reference, [n][3]."""
+    L = ref_lib()
+    n = L.ref_spiral(*args, None, None, None, 0)
+    px, py, pz = (np.empty(max(n, 1)) for _ in range(3))
+    L.ref_spiral(*args, _p(px), _p(py), _p(pz), n)
+    return np.stack([px[:n], py[:n], pz[:n]], 1)
+
+
+def ref_default_trajectory(rx: bool) -> np.ndarray:
+    """generate{Transmitter,Receiver}Trajectory(initializeRadar{Tx,Rx}()) of the compiled reference."""
+    L = ref_lib()
+    n = L.ref_default_trajectory(int(rx), None, 0)
+    out = np.empty((n, 3))
+    L.ref_default_trajectory(int(rx), _p(out), n)
+    return out
+
+
+def ref_grid(pos, dxy, lxy, dz, lz):
+    """crearGrid axes of the compiled reference."""
+    L = ref_lib()
+    n = (C.c_int * 3)()
+    pos = np.ascontiguousarray(pos, np.float64)
+    buf = np.empty(1 << 20)
+    L.ref_grid(_p(pos), dxy, lxy, dz, lz, _p(buf), len(buf), n)
+    a, b, c = n[0], n[1], n[2]
+    return buf[:a].copy(), buf[a:a + b].copy(), buf[a + b:a + b + c].copy()
+
+
+def ref_load_dem(path):
+    """loadDEM (load_dem_simple.cpp) of the compiled reference: X_vec, Y_vec,
+    Z_DEM flattened (rows of Z_DEM[0]'s length) and the four sizes."""
+    L = ref_lib()
+    n = (C.c_int * 4)()
+    cap = 1 << 20
+    x, y, z = np.empty(cap), np.empty(cap), np.empty(cap)
+    L.ref_load_dem(str(path).encode(), _p(x), cap, _p(y), cap, _p(z), cap, n)
+    nx, ny, zr, zc = n[0], n[1], n[2], n[3]
+    return x[:nx].copy(), y[:ny].copy(), z[:zr * zc].copy().reshape(zr, zc), (nx, ny, zr, zc)
+
+
+# ---- scenario generators and map projection (independent restatements) ----
+
+def spiral(N, r_minor, r_major, h_high, h_low, vt, prf, mirror=False) -> np.ndarray:
+    """funcao_espiral (funcao_espiral.cpp:5-49) in scalar Python: the same
+    operations in the same order, glibc's log and sincos."""
+    import math
+    m = C.CDLL("libm.so.6")
+    m.sincos.argtypes = [C.c_double, _dp, _dp]
+    sn, cs = C.c_double(), C.c_double()
+    rs, re_ = r_major, r_minor
+    vr = vt * (math.log(rs) - math.log(re_)) / (N * 2 * math.pi)
+    T = (rs - re_) / vr if vr != 0 else float("nan")
+    vd = (h_high - h_low) / T
+    dt = 1.0 / prf
+    out = []
+    t = 0.0
+    while t <= T:
+        theta = (vt / vr) * (math.log(rs) - math.log(rs - vr * t))
+        R = rs - vr * t
+        # g++ -O3 (the reference's build) folds cos(theta), sin(theta) into one sincos call
+        m.sincos(theta, C.byref(sn), C.byref(cs))
+        x, y = R * cs.value, R * sn.value
+        out.append((-x if mirror else x, -y if mirror else y, h_high - vd * t))
+        t += dt
+    return np.array(out).reshape(-1, 3)
+
+
+def grid_axis(center, step, half) -> np.ndarray:
+    """One crearGrid axis (crear_grid.cpp:21-34): a running sum."""
+    out = []
+    v = center - half
+    while v <= center + half + step / 2:
+        out.append(v)
+        v += step
+    return np.array(out)
+
+
+WGS84 = (6378137.0, 1 / 298.257223563)
+
+
+def meridian_arc_quad(lat_deg, a=WGS84[0], f=WGS84[1]) -> float:
+    """Meridian arc from the equator by numerical quadrature of the meridian
+    radius of curvature a(1-e^2)/(1-e^2 sin^2)^1.5."""
+    from scipy.integrate import quad
+    e2 = f * (2 - f)
+    val, _ = quad(lambda p: a * (1 - e2) / (1 - e2 * np.sin(p) ** 2) ** 1.5, 0.0, np.radians(lat_deg),
+                  epsabs=1e-9, epsrel=1e-13, limit=200)
+    return val
+
+
+def utm_snyder(lat_deg, lon_deg, lon0_deg, a=WGS84[0], f=WGS84[1]):
+    """Transverse Mercator by Snyder's series in A = (lon - lon0) cos(lat)
+    (USGS Professional Paper 1395, eqs. 3-21, 4-20, 8-9, 8-10, 8-12..8-15),
+    UTM scale 0.9996, false easting 500 km, false northing 10,000 km south."""
+    k0 = 0.9996
+    e2 = f * (2 - f)
+    ep2 = e2 / (1 - e2)
+    phi = np.radians(lat_deg)
+    lam = np.radians(lon_deg - lon0_deg)
+    N = a / np.sqrt(1 - e2 * np.sin(phi) ** 2)
+    T = np.tan(phi) ** 2
+    Cc = ep2 * np.cos(phi) ** 2
+    A = lam * np.cos(phi)
+    M = a * ((1 - e2 / 4 - 3 * e2 ** 2 / 64 - 5 * e2 ** 3 / 256) * phi
+             - (3 * e2 / 8 + 3 * e2 ** 2 / 32 + 45 * e2 ** 3 / 1024) * np.sin(2 * phi)
+             + (15 * e2 ** 2 / 256 + 45 * e2 ** 3 / 1024) * np.sin(4 * phi)
+             - (35 * e2 ** 3 / 3072) * np.sin(6 * phi))
+    x = k0 * N * (A + (1 - T + Cc) * A ** 3 / 6 + (5 - 18 * T + T ** 2 + 72 * Cc - 58 * ep2) * A ** 5 / 120)
+    y = k0 * (M + N * np.tan(phi) * (A ** 2 / 2 + (5 - T + 9 * Cc + 4 * Cc ** 2) * A ** 4 / 24
+                                     + (61 - 58 * T + T ** 2 + 600 * Cc - 330 * ep2) * A ** 6 / 720))
+    return 500000.0 + x, (10000000.0 if lat_deg < 0 else 0.0) + y

@@ -1,7 +1,9 @@
 // ref_shim.cpp -- extern "C" entry points over the reference's own hot-path
 // functions, for building oracle/_ref/libref.so from the unmodified reference
 // sources (src/raw_data_processing.cpp, src/fft_utils.cpp,
-// src/interpolation.cpp, src/project_paths.cpp).
+// src/interpolation.cpp, src/project_paths.cpp, and the scenario generators
+// src/funcao_espiral.cpp, src/trajectory_generator.cpp, src/radar_init.cpp,
+// src/crear_grid.cpp, src/load_dem_simple.cpp).
 //
 // TEST INFRASTRUCTURE ONLY: used to pin the C restatement in
 // oracle/bsar_oracle.c against the reference itself (tests/golden/) and, on
@@ -10,6 +12,11 @@
 #include "raw_data_processing.h"
 #include "fft_utils.h"
 #include "interpolation.h"
+#include "funcao_espiral.h"
+#include "trajectory_generator.h"
+#include "radar_init.h"
+#include "crear_grid.h"
+#include "load_dem.h"
 
 #include <cstring>
 #include <vector>
@@ -152,6 +159,53 @@ void ref_process_raw(const double* x, int nx, const double* y, int ny, const dou
             raw[2 * k] = res.rawData[row][i].real(); raw[2 * k + 1] = res.rawData[row][i].imag();
             root[2 * k] = res.rootData[row][i].real(); root[2 * k + 1] = res.rootData[row][i].imag();
         }
+}
+
+// funcao_espiral: returns the point count, fills up to cap positions
+int64_t ref_spiral(double N, double r_minor, double r_major, double h_high, double h_low, double vt, double prf,
+                   double* px, double* py, double* pz, int64_t cap)
+{
+    TrajectoryPoints t = funcao_espiral(N, r_minor, r_major, h_high, h_low, vt, prf);
+    for (size_t i = 0; i < t.Px.size() && (int64_t)i < cap; i++) { px[i] = t.Px[i]; py[i] = t.Py[i]; pz[i] = t.Pz[i]; }
+    return (int64_t)t.Px.size();
+}
+
+// generateTransmitterTrajectory(initializeRadarTx()) (rx = 0) or
+// generateReceiverTrajectory(initializeRadarRx()) (rx = 1), out[n][3]
+int64_t ref_default_trajectory(int rx, double* out, int64_t cap)
+{
+    TrajectoryPoints t = rx ? generateReceiverTrajectory(initializeRadarRx())
+                            : generateTransmitterTrajectory(initializeRadarTx());
+    for (size_t i = 0; i < t.Px.size() && (int64_t)i < cap; i++) {
+        out[3 * i] = t.Px[i]; out[3 * i + 1] = t.Py[i]; out[3 * i + 2] = t.Pz[i];
+    }
+    return (int64_t)t.Px.size();
+}
+
+// crearGrid around pos: axes written back to back into ax (cap values), sizes to n[3]
+void ref_grid(const double* pos, double dxy, double lxy, double dz, double lz, double* ax, int cap, int* n)
+{
+    Target t = initializeTarget();
+    updatePointToProc(t, pos[0], pos[1], pos[2]);
+    Grid3D g = crearGrid(t, dxy, lxy, dz, lz);
+    n[0] = (int)g.gridToProc.xAxis.size(); n[1] = (int)g.gridToProc.yAxis.size(); n[2] = (int)g.gridToProc.zAxis.size();
+    int k = 0;
+    for (const auto* v : {&g.gridToProc.xAxis, &g.gridToProc.yAxis, &g.gridToProc.zAxis})
+        for (double a : *v) if (k < cap) ax[k++] = a;
+}
+
+// loadDEM (load_dem_simple.cpp): sizes to n[4] = (nx, ny, zrows, zcols of row 0);
+// arrays filled up to their capacities
+void ref_load_dem(const char* path, double* x, int capx, double* y, int capy, double* z, int capz, int* n)
+{
+    ProcessedDEM d = loadDEM(path);
+    n[0] = (int)d.X_vec.size(); n[1] = (int)d.Y_vec.size(); n[2] = (int)d.Z_DEM.size();
+    n[3] = d.Z_DEM.empty() ? 0 : (int)d.Z_DEM[0].size();
+    for (int i = 0; i < n[0] && i < capx; i++) x[i] = d.X_vec[i];
+    for (int i = 0; i < n[1] && i < capy; i++) y[i] = d.Y_vec[i];
+    int k = 0;
+    for (const auto& row : d.Z_DEM)
+        for (double v : row) if (k < capz) z[k++] = v;
 }
 
 }  // extern "C"

@@ -10,8 +10,13 @@ outputs are whatever the reference's own code returns for them:
                    createCHIRP, processRawData  (through oracle/ref_shim.cpp)
   ref_bp_*.npz     output_3D_matrix.bin written by the unmodified
                    proc_test_script run on a generated build/output/proc/ set
+  ref_scenario.npz the file set the unmodified gs_test_script writes for its
+                   default scenario (radar_init.cpp constants, spiral
+                   trajectories, crearGrid, loadDEM of a .tif name): the small
+                   CSV/text files whole, the trajectory CSVs as SHA-256, the
+                   three complex matrices on a subset of pulses
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--only scenario]
 """
 from __future__ import annotations
 
@@ -220,14 +225,62 @@ def bp_case(rng, tag, n, S, grid, data_kind):
     print(f"wrote ref_bp_{tag}.npz ({len(blob)} bytes of image)")
 
 
+SCENARIO_PULSES = [0, 1, 2, 3, 1000, 2000, 3000, 4000, 5000, 6000, 6685, 6686, 6687, 6688]
+SCENARIO_TEXT = ["DEM_X_vec.csv", "DEM_Y_vec.csv", "DEM_Z_matrix.csv", "grid_xAxis.csv", "grid_yAxis.csv",
+                 "grid_zAxis.csv", "target_position.csv", "environment.txt"]
+
+
+def scenario_case():
+    """gs_test_script's default run (it writes build/output/ and plot_data/
+    under its working directory)."""
+    import hashlib
+    work = Path(tempfile.mkdtemp(prefix="bsar_golden_gs_"))
+    try:
+        r = subprocess.run([str(O.REF_GS)], cwd=work, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        proc = work / "build" / "output" / "proc"
+        res = {f"text_{n}": np.frombuffer((proc / n).read_bytes(), np.uint8) for n in SCENARIO_TEXT}
+        for n in ("Tx_positions.csv", "Rx_positions.csv"):
+            b = (proc / n).read_bytes()
+            res[f"sha256_{n}"] = np.frombuffer(hashlib.sha256(b).digest(), np.uint8)
+            res[f"lines_{n}"] = np.array(b.decode().count("\n"))
+        for n in ("rootData", "rawData", "auxData"):
+            blob = (proc / f"{n}_complex.bin").read_bytes()
+            hdr, ndim = blob[:28], struct.unpack("<I", blob[:4])[0]
+            dims = struct.unpack("<%dQ" % ndim, blob[28:28 + 8 * ndim])
+            data = np.frombuffer(blob[28 + 8 * ndim:], np.complex128).reshape(dims)   # [S][N], r*N + p
+            res[f"{n}_header"] = np.frombuffer(blob[:28 + 8 * ndim], np.uint8)
+            res[f"{n}_pulses"] = np.ascontiguousarray(data[:, SCENARIO_PULSES].T)       # [P][S]
+            if n == "auxData":
+                nz = np.abs(data) > 0
+                res["aux_bins"] = np.where(nz.any(0), nz.argmax(0), -1).astype(np.int16)  # per pulse
+        res["pulses"] = np.array(SCENARIO_PULSES)
+        res["n_pulses"] = np.array(dims[1])
+        cxx = subprocess.run(["g++", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+        res["manifest"] = np.array(f"oracle/_ref/gs_test_script built by oracle/Makefile ({cxx}, -O3 -fopenmp), "
+                                   f"run with no arguments in an empty directory")
+    finally:
+        shutil.rmtree(work)
+    np.savez_compressed(OUT / "ref_scenario.npz", **res)
+    print("wrote ref_scenario.npz")
+
+
 def main():
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", choices=["scenario"], help="write only this fixture (the others share one seeded "
+                    "stream and are written together)")
+    a = ap.parse_args()
     O.build(ref=True)
     if O.ref_lib() is None:
         sys.exit("compiled reference missing: oracle/_ref was not built")
-    rng = np.random.default_rng(20261203)
-    kernels(rng)
-    bp_case(rng, "random", n=16, S=128, grid=(5, 4, 3), data_kind="random")
-    bp_case(rng, "target", n=24, S=128, grid=(6, 5, 4), data_kind="target")
+    if a.only is None:
+        rng = np.random.default_rng(20261203)
+        kernels(rng)
+        bp_case(rng, "random", n=16, S=128, grid=(5, 4, 3), data_kind="random")
+        bp_case(rng, "target", n=24, S=128, grid=(6, 5, 4), data_kind="target")
+    scenario_case()
 
 
 if __name__ == "__main__":
