@@ -16,6 +16,7 @@
 
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 
 #include "bsar_fft.h"
@@ -38,123 +39,206 @@ __device__ __forceinline__ V cadd(V a, V b) { return V{a.x + b.x, a.y + b.y}; }
 template <typename V>
 __device__ __forceinline__ V csub(V a, V b) { return V{a.x - b.x, a.y - b.y}; }
 
-// Group of 2^K elements e_m = base + m h of K consecutive radix-2 DIT stages
-// s .. s+K-1 (h = 2^(s-1), j = q mod h): stage s+t pairs (e_m, e_m+2^t) with
-// twiddle exp(-i pi (j + (m mod 2^t) h) / (2^t h)) = W^(2^(K-1-t)) *
-// exp(-i pi (m mod 2^t) / 2^t), W = exp(-i pi j / (2^(K-1) h)) = tw[j S / (2^K h)]
-// where tw[k] = exp(-2 pi i k / S).  Inverse (DIF) groups run the stages in
-// the opposite order with conjugate twiddles.
-template <typename T, int K, bool kInv>
-__device__ __forceinline__ void fft_group(typename C2<T>::type* x, int s, int q, int logS,
-                                          const typename C2<T>::type* __restrict__ tw)
+// exp(sgn i pi r / d) for d <= 8 (multiples of a sixteenth turn), exact for
+// r = 0 and the quarter turns.
+template <typename T>
+__device__ __forceinline__ typename C2<T>::type rot16(int r, int d, T sgn)
+{
+    using V = typename C2<T>::type;
+    constexpr T c1 = (T)0.92387953251128675613;   // cos(pi/8)
+    constexpr T s1 = (T)0.38268343236508977173;   // sin(pi/8)
+    constexpr T c2 = (T)0.70710678118654752440;   // cos(pi/4)
+    const int k = r * (8 / d);                    // angle k pi / 8, k < 8
+    switch (k) {
+    case 0: return V{(T)1, (T)0};
+    case 1: return V{c1, sgn * s1};
+    case 2: return V{c2, sgn * c2};
+    case 3: return V{s1, sgn * c1};
+    case 4: return V{(T)0, sgn};
+    case 5: return V{-s1, sgn * c1};
+    case 6: return V{-c2, sgn * c2};
+    default: return V{-c1, sgn * s1};
+    }
+}
+
+// K consecutive radix-2 stages on the 2^K elements v[m] = x[e0 + m h] (h =
+// 2^lh, j = e0 mod h).  The stage at distance 2^t h pairs v[m], v[m + 2^t]
+// with twiddle exp(sgn i pi (j + (m mod 2^t) h) / (2^t h)) = W^(2^(K-1-t)) *
+// exp(sgn i pi (m mod 2^t) / 2^t), W = exp(sgn i pi j / (2^(K-1) h)) =
+// tw[j S / (2^K h)] (conjugated for the inverse), tw[k] = exp(-2 pi i k / S).
+// kDif: decimation in frequency (distances descending, twiddle after the
+// difference: natural order in, bit-reversed out); else decimation in time
+// (distances ascending, twiddle on the second input: bit-reversed in,
+// natural out).
+// kUnit: j = 0 (W = 1), the twiddles are the constant rotations alone.
+template <typename T, int K, bool kDif, bool kInv, bool kUnit = false>
+__device__ __forceinline__ void butterflies(typename C2<T>::type* v, int j, int lh, int logS,
+                                            const typename C2<T>::type* __restrict__ tw)
 {
     using V = typename C2<T>::type;
     constexpr int G = 1 << K;
-    // forward: h = 2^(s-1) is the smallest distance; inverse: s is the top stage, h = 2^(s-K)
-    const int lh = kInv ? (s - K) : (s - 1);
-    const int h = 1 << lh;
-    const int j = q & (h - 1);
-    const int base = ((q >> lh) << (lh + K)) + j;
-    V w = tw[(int64_t)j << (logS - K - lh)];                         // exp(-i pi j / (2^(K-1) h))
-    if (kInv) w.y = -w.y;
-    // powers: wt[t] = W^(2^(K-1-t)), t = K-1 .. 0
-    V wt[K];
-    wt[K - 1] = w;
-#pragma unroll
-    for (int t = K - 2; t >= 0; --t) wt[t] = cmul(wt[t + 1], wt[t + 1]);
-    V v[G];
-#pragma unroll
-    for (int m = 0; m < G; ++m) v[m] = x[base + m * h];
     const T sgn = kInv ? (T)1 : (T)-1;
+    V wt[K];
+    if (kUnit) {
+#pragma unroll
+        for (int t = 0; t < K; ++t) wt[t] = V{(T)1, (T)0};
+    } else {
+        V w = tw[(int64_t)j << (logS - K - lh)];
+        if (kInv) w.y = -w.y;
+        wt[K - 1] = w;
+#pragma unroll
+        for (int t = K - 2; t >= 0; --t) wt[t] = cmul(wt[t + 1], wt[t + 1]);
+    }
 #pragma unroll
     for (int tt = 0; tt < K; ++tt) {
-        const int t = kInv ? (K - 1 - tt) : tt;                      // DIT: distance 2^t ascending; DIF: descending
+        const int t = kDif ? (K - 1 - tt) : tt;
         const int d = 1 << t;
 #pragma unroll
         for (int m = 0; m < G; ++m) {
             if (m & d) continue;
             const int r = m & (d - 1);
-            // exp(sgn i pi r / d), d <= 4: multiples of an eighth turn
-            constexpr T c8 = (T)0.70710678118654752440;
-            V rot;
-            if (r == 0) rot = V{(T)1, (T)0};
-            else if (2 * r == d) rot = V{(T)0, sgn};
-            else if (r == 1) rot = V{c8, sgn * c8};
-            else rot = V{-c8, sgn * c8};                               // r = 3, d = 4
-            const V tw_m = (r == 0) ? wt[t] : cmul(wt[t], rot);
-            if (!kInv) {
-                const V u = v[m], b = cmul(v[m + d], tw_m);
-                v[m] = cadd(u, b);
-                v[m + d] = csub(u, b);
-            } else {
+            const V tw_m = kUnit ? rot16<T>(r, d, sgn) : ((r == 0) ? wt[t] : cmul(wt[t], rot16<T>(r, d, sgn)));
+            const bool trivial = kUnit && r == 0;
+            if (kDif) {
                 const V u = v[m], b = v[m + d];
                 v[m] = cadd(u, b);
-                v[m + d] = cmul(csub(u, b), tw_m);
+                v[m + d] = trivial ? csub(u, b) : cmul(csub(u, b), tw_m);
+            } else {
+                const V u = v[m], b = trivial ? v[m + d] : cmul(v[m + d], tw_m);
+                v[m] = cadd(u, b);
+                v[m + d] = csub(u, b);
             }
         }
     }
-#pragma unroll
-    for (int m = 0; m < G; ++m) x[base + m * h] = v[m];
 }
 
-// Fused correlation: out[c] = IFFT(FFT(in[c]) * H) for one column c per block.
-// H is the (already conjugated when correlating) reference spectrum, length
-// S, natural order; tw[k] = exp(-2 pi i k / S), k < S / 2.  The column is
-// loaded bit-reversed, transformed by radix-2 DIT stages taken up to three at
-// a time in registers (one LDS round trip per group), multiplied by H,
-// inverse transformed by DIF stages three at a time, and stored
-// bit-reversed / S.
+// LDS layout: element e at e + e / 32 (one pad slot per 32 keeps the strided
+// accesses of every pass within 2-4 way bank conflicts).
+__device__ __forceinline__ int lds_pad(int e) { return e + (e >> 5); }
+
+// One work item q of a K-stage LDS pass whose smallest distance is 2^lh.
+template <typename T, int K, bool kDif, bool kInv>
+__device__ __forceinline__ void lds_group(typename C2<T>::type* x, int lh, int q, int logS,
+                                          const typename C2<T>::type* __restrict__ tw)
+{
+    using V = typename C2<T>::type;
+    constexpr int G = 1 << K;
+    const int h = 1 << lh;
+    const int j = q & (h - 1);
+    const int e0 = ((q >> lh) << (lh + K)) + j;
+    V v[G];
+#pragma unroll
+    for (int m = 0; m < G; ++m) v[m] = x[lds_pad(e0 + m * h)];
+    butterflies<T, K, kDif, kInv>(v, j, lh, logS, tw);
+#pragma unroll
+    for (int m = 0; m < G; ++m) x[lds_pad(e0 + m * h)] = v[m];
+}
+
+// The innermost Kb stages (distances 2^(Kb-1) .. 1) of the forward DIF, the
+// product with the spectrum and the same stages of the inverse DIT in one LDS
+// pass: the two transforms meet on the same 2^Kb contiguous elements.
+template <typename T, int Kb>
+__device__ __forceinline__ void lds_turn(typename C2<T>::type* x, int q, int logS,
+                                         const typename C2<T>::type* __restrict__ Hbr,
+                                         const typename C2<T>::type* __restrict__ tw)
+{
+    using V = typename C2<T>::type;
+    constexpr int G = 1 << Kb;
+    const int e0 = q << Kb;
+    V v[G];
+#pragma unroll
+    for (int m = 0; m < G; ++m) v[m] = x[lds_pad(e0 + m)];
+    butterflies<T, Kb, true, false, true>(v, 0, 0, logS, tw);
+#pragma unroll
+    for (int m = 0; m < G; ++m) v[m] = cmul(v[m], Hbr[e0 + m]);
+    butterflies<T, Kb, false, true, true>(v, 0, 0, logS, tw);
+#pragma unroll
+    for (int m = 0; m < G; ++m) x[lds_pad(e0 + m)] = v[m];
+}
+
+// The stages below distance 2^top in LDS (top = log2 of the register pass's
+// smallest distance, >= 1): forward DIF in radix-16 passes from the top, the
+// innermost Kb = top mod 4 (or 4) stages fused with the product and the
+// inverse's innermost stages, then the inverse DIT in radix-16 passes
+// outwards.
+template <typename T, int Kb>
+__device__ __forceinline__ void lds_filter(typename C2<T>::type* x, int top, int logS,
+                                           const typename C2<T>::type* __restrict__ Hbr,
+                                           const typename C2<T>::type* __restrict__ tw)
+{
+    const int S = 1 << logS;
+    for (int s = top; s > Kb; s -= 4) {
+#pragma unroll 1
+        for (int q = threadIdx.x; q < (S >> 4); q += blockDim.x) lds_group<T, 4, true, false>(x, s - 4, q, logS, tw);
+        __syncthreads();
+    }
+#pragma unroll 1
+    for (int q = threadIdx.x; q < (S >> Kb); q += blockDim.x) lds_turn<T, Kb>(x, q, logS, Hbr, tw);
+    __syncthreads();
+    for (int s = Kb; s < top; s += 4) {
+#pragma unroll 1
+        for (int q = threadIdx.x; q < (S >> 4); q += blockDim.x) lds_group<T, 4, false, true>(x, s, q, logS, tw);
+        __syncthreads();
+    }
+}
+
+// Threads per block at most: 16384 complex64 or 8192 complex128 samples (the
+// LDS budget) over 16 samples per thread.
 template <typename T>
-__global__ __launch_bounds__(1024) void k_fft_filter_lds(const typename C2<T>::type* __restrict__ in,
+constexpr int fft_max_threads() { return (int)(kFftLdsMax / sizeof(typename C2<T>::type) / 16); }
+
+// Fused filter: out[c] = IFFT(FFT(in[c]) .* H) for one column c per block of
+// S / 2^KP threads (Kb = the innermost LDS stages, (logS - KP) mod 4 or 4).  Thread t owns samples t + i S/2^KP (i < 2^KP): it loads
+// them coalesced, runs the KP largest-distance DIF stages in registers and
+// stores them to LDS; the remaining stages, the product with the spectrum in
+// bit-reversed order (Hbr = H permuted) and the inverse's matching stages
+// run in LDS; the thread then reads its samples back, runs the KP
+// largest-distance DIT stages of the inverse in registers and stores the
+// samples / S coalesced.  tw[k] = exp(-2 pi i k / S), k < S / 2.
+template <typename T, int KP, int Kb>
+__global__ __launch_bounds__(fft_max_threads<T>()) void k_fft_filter_lds(const typename C2<T>::type* __restrict__ in,
                                                          typename C2<T>::type* __restrict__ out,
-                                                         const typename C2<T>::type* __restrict__ H,
+                                                         const typename C2<T>::type* __restrict__ Hbr,
                                                          const typename C2<T>::type* __restrict__ tw, int logS)
 {
     using V = typename C2<T>::type;
+    constexpr int P = 1 << KP;
     extern __shared__ unsigned char smem[];
     V* x = reinterpret_cast<V*>(smem);
     const int S = 1 << logS;
+    const int nt = S >> KP;                 // == blockDim.x
+    const int t = threadIdx.x;
+    const int top = logS - KP;              // register stages have distances 2^top .. S/2
     const int64_t col = blockIdx.x;
-    const V* src = in + col * S;
-    for (int k = threadIdx.x; k < S; k += blockDim.x) x[bitrev(k, logS)] = src[k];
-    __syncthreads();
-    // forward DIT: groups of 3 stages, the remainder (1 or 2) first
-    int s = 1;
-    const int rem = logS % 3;
-    if (rem == 1) {
-        for (int q = threadIdx.x; q < S / 2; q += blockDim.x) fft_group<T, 1, false>(x, s, q, logS, tw);
+    V v[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) v[i] = in[col * S + t + i * nt];
+    butterflies<T, KP, true, false>(v, t, top, logS, tw);
+    if (top == 0) {
+        // S = 2^KP: the registers hold the whole spectrum (bit-reversed)
+#pragma unroll
+        for (int i = 0; i < P; ++i) v[i] = cmul(v[i], Hbr[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < P; ++i) x[lds_pad(t + i * nt)] = v[i];
         __syncthreads();
-        s += 1;
-    } else if (rem == 2) {
-        for (int q = threadIdx.x; q < S / 4; q += blockDim.x) fft_group<T, 2, false>(x, s, q, logS, tw);
-        __syncthreads();
-        s += 2;
+        lds_filter<T, Kb>(x, top, logS, Hbr, tw);
+#pragma unroll
+        for (int i = 0; i < P; ++i) v[i] = x[lds_pad(t + i * nt)];
     }
-    for (; s + 2 <= logS; s += 3) {
-        for (int q = threadIdx.x; q < S / 8; q += blockDim.x) fft_group<T, 3, false>(x, s, q, logS, tw);
-        __syncthreads();
-    }
-    for (int k = threadIdx.x; k < S; k += blockDim.x) x[k] = cmul(x[k], H[k]);
-    __syncthreads();
-    // inverse DIF: groups of 3 stages from the top (s = top stage), the
-    // remainder (stages rem .. 1) last
-    for (s = logS; s >= rem + 3; s -= 3) {
-        for (int q = threadIdx.x; q < S / 8; q += blockDim.x) fft_group<T, 3, true>(x, s, q, logS, tw);
-        __syncthreads();
-    }
-    if (rem == 1) {
-        for (int q = threadIdx.x; q < S / 2; q += blockDim.x) fft_group<T, 1, true>(x, s, q, logS, tw);
-        __syncthreads();
-    } else if (rem == 2) {
-        for (int q = threadIdx.x; q < S / 4; q += blockDim.x) fft_group<T, 2, true>(x, s, q, logS, tw);
-        __syncthreads();
-    }
+    butterflies<T, KP, false, true>(v, t, top, logS, tw);
     const T inv = (T)1 / (T)S;
-    V* dst = out + col * S;
-    for (int k = threadIdx.x; k < S; k += blockDim.x) {
-        const V a = x[bitrev(k, logS)];
-        dst[k] = V{a.x * inv, a.y * inv};
-    }
+#pragma unroll
+    for (int i = 0; i < P; ++i) out[col * S + t + i * nt] = V{v[i].x * inv, v[i].y * inv};
+}
+
+// Hbr[k] = H[bitrev(k)].
+template <typename T>
+__global__ void k_bitrev_copy(const typename C2<T>::type* __restrict__ H, typename C2<T>::type* __restrict__ Hbr,
+                              int logS)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < (1 << logS)) Hbr[k] = H[bitrev((uint32_t)k, logS)];
 }
 
 // tw[k] = exp(-2 pi i k / S), k < S / 2, from FP64 sincospi.
@@ -262,8 +346,24 @@ static hipError_t fft_global(typename C2<T>::type* a, typename C2<T>::type* tmp,
     return hipGetLastError();
 }
 
+// LDS of one column with one pad slot per 32 elements
 template <typename T>
-static size_t lds_bytes(int S) { return (size_t)S * sizeof(typename C2<T>::type); }
+static size_t lds_bytes(int S) { return (size_t)(S + S / 32 + 1) * sizeof(typename C2<T>::type); }
+
+template <typename T, int KP, int Kb>
+static hipError_t launch_lds_filter(typename C2<T>::type* data, const typename C2<T>::type* Hbr,
+                                    const typename C2<T>::type* tw, int logS, int64_t ncols, hipStream_t st)
+{
+    const int S = 1 << logS;
+    if ((S >> KP) > fft_max_threads<T>()) return hipErrorInvalidValue;
+    const size_t lb = lds_bytes<T>(S);
+    hipError_t e = hipFuncSetAttribute((const void*)k_fft_filter_lds<T, KP, Kb>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_fft_filter_lds<T, KP, Kb>), dim3((unsigned)ncols), dim3(S >> KP), lb, st, data, data, Hbr,
+                       tw, logS);
+    return hipGetLastError();
+}
 
 // Twiddle tables tw[k] = exp(-2 pi i k / S), k < S / 2, one per (device, S,
 // precision), built on first use and kept for the life of the process.
@@ -299,18 +399,29 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
     using V = typename C2<T>::type;
     if (S < 2 || (S & (S - 1)) || ncols <= 0) return hipErrorInvalidValue;
     const int logS = ilog2(S);
-    const size_t lb = lds_bytes<T>(S);
-    if (lb <= kFftLdsMax) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_fft_filter_lds<T>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
-        if (e != hipSuccess) return e;
+    if ((size_t)S * sizeof(V) <= kFftLdsMax) {
+        // S / 2^KP <= 1024 threads: S <= 16384 (complex64) / 8192 (complex128)
+        if (!tmp) return hipErrorInvalidValue;
         const V* tw = nullptr;
-        if ((e = twiddle_table<T>(S, st, &tw)) != hipSuccess) return e;
-        // one thread per 8-point group of the radix-8 passes
-        const int threads = S / 8 >= 1024 ? 1024 : (S / 8 < 64 ? 64 : S / 8);
-        hipLaunchKernelGGL(k_fft_filter_lds<T>, dim3((unsigned)ncols), dim3(threads), lb, st, data, data, H, tw,
-                           logS);
-        return hipGetLastError();
+        hipError_t e = twiddle_table<T>(S, st, &tw);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_bitrev_copy<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        // S <= 16: the register pass holds the column; else 16 samples per
+        // thread and the innermost (logS - 4) mod 4 (or 4) stages fused
+        switch (logS) {
+        case 1: return launch_lds_filter<T, 1, 1>(data, tmp, tw, logS, ncols, st);
+        case 2: return launch_lds_filter<T, 2, 1>(data, tmp, tw, logS, ncols, st);
+        case 3: return launch_lds_filter<T, 3, 1>(data, tmp, tw, logS, ncols, st);
+        case 4: return launch_lds_filter<T, 4, 1>(data, tmp, tw, logS, ncols, st);
+        default:
+            switch ((logS - 4) % 4) {
+            case 1: return launch_lds_filter<T, 4, 1>(data, tmp, tw, logS, ncols, st);
+            case 2: return launch_lds_filter<T, 4, 2>(data, tmp, tw, logS, ncols, st);
+            case 3: return launch_lds_filter<T, 4, 3>(data, tmp, tw, logS, ncols, st);
+            default: return launch_lds_filter<T, 4, 4>(data, tmp, tw, logS, ncols, st);
+            }
+        }
     }
     V* res = nullptr;
     hipError_t e = fft_global<T>(data, tmp, S, ncols, -1, st, &res);

@@ -56,18 +56,16 @@ extern "C" int bsar_range_compress(bsar_ctx* c, const double* sig, int S, int N,
     double2* H = nullptr;
     BSAR_CHECK_HIP(c, fft_forward((double2*)dref.p, (double2*)dtmp1.p, (int)P, 1, c->stream, &H));
     if (conj) BSAR_CHECK_HIP(c, conj_inplace(H, (int64_t)P, c->stream));
-    const bool need_tmp_fp64 = P * sizeof(double2) > kFftLdsMax;
-    const bool need_tmp_fp32 = P * sizeof(float2) > kFftLdsMax;
     BSAR_CHECK_HIP(c, hipMalloc(&data.p, n * sizeof(double2)));
     BSAR_CHECK_HIP(c, hipMemcpyAsync(data.p, src, n * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     if (precision == BSAR_FP64) {
-        if (need_tmp_fp64) BSAR_CHECK_HIP(c, hipMalloc(&tmp.p, n * sizeof(double2)));
+        BSAR_CHECK_HIP(c, hipMalloc(&tmp.p, fft_filter_tmp_elems(P, N, sizeof(double2)) * sizeof(double2)));
         BSAR_CHECK_HIP(c, fft_filter((double2*)data.p, (double2*)tmp.p, H, (int)P, N, c->stream));
     } else {
         Buf d32, h32;
         BSAR_CHECK_HIP(c, hipMalloc(&d32.p, n * sizeof(float2)));
         BSAR_CHECK_HIP(c, hipMalloc(&h32.p, P * sizeof(float2)));
-        if (need_tmp_fp32) BSAR_CHECK_HIP(c, hipMalloc(&tmp.p, n * sizeof(float2)));
+        BSAR_CHECK_HIP(c, hipMalloc(&tmp.p, fft_filter_tmp_elems(P, N, sizeof(float2)) * sizeof(float2)));
         BSAR_CHECK_HIP(c, to_f32((const double2*)data.p, (float2*)d32.p, (int64_t)n, c->stream));
         BSAR_CHECK_HIP(c, to_f32(H, (float2*)h32.p, (int64_t)P, c->stream));
         BSAR_CHECK_HIP(c, fft_filter((float2*)d32.p, (float2*)tmp.p, (const float2*)h32.p, (int)P, N, c->stream));
@@ -107,7 +105,7 @@ extern "C" int bsar_range_compress_device(bsar_ctx* c, void* d_sig, int S, int N
     BSAR_CHECK_HIP(c, fft_forward((double2*)dref.p, (double2*)dtmp1.p, (int)P, 1, c->stream, &H));
     if (conj) BSAR_CHECK_HIP(c, conj_inplace(H, (int64_t)P, c->stream));
     const size_t esz = precision == BSAR_FP64 ? sizeof(double2) : sizeof(float2);
-    if (P * esz > kFftLdsMax) BSAR_CHECK_HIP(c, hipMalloc(&tmp.p, n * esz));
+    BSAR_CHECK_HIP(c, hipMalloc(&tmp.p, fft_filter_tmp_elems(P, N, esz) * esz));
     if (precision == BSAR_MIXED) {
         BSAR_CHECK_HIP(c, hipMalloc(&h32.p, P * sizeof(float2)));
         BSAR_CHECK_HIP(c, to_f32(H, (float2*)h32.p, (int64_t)P, c->stream));

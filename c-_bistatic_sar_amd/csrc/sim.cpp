@@ -75,7 +75,7 @@ extern "C" int bsar_process_raw(bsar_ctx* c, const bsar_dataset* ds, const doubl
         BSAR_CHECK_HIP(c, hipMemcpyAsync(dref.p, ref.data(), (size_t)S * sizeof(double2), hipMemcpyHostToDevice, c->stream));
         double2* H = nullptr;
         BSAR_CHECK_HIP(c, fft_forward((double2*)dref.p, (double2*)dtmp.p, S, 1, c->stream, &H));
-        if ((size_t)S * sizeof(double2) > kFftLdsMax) BSAR_CHECK_HIP(c, hipMalloc(&dtmp2.p, cells * sizeof(double2)));
+        BSAR_CHECK_HIP(c, hipMalloc(&dtmp2.p, fft_filter_tmp_elems(S, n, sizeof(double2)) * sizeof(double2)));
         BSAR_CHECK_HIP(c, fft_filter((double2*)daux.p, (double2*)dtmp2.p, H, S, n, c->stream));
         if (raw) BSAR_CHECK_HIP(c, hipMemcpyAsync(raw, daux.p, cells * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
         if (root) {

@@ -79,6 +79,9 @@ int main(int argc, char** argv)
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 2; }
     }
     try {
+        using clk = std::chrono::steady_clock;
+        const auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
+        const auto t_read = clk::now();
         double n1, n2;
         read_environment(dir + "/environment.txt", n1, n2);
         fa::MatrixInfo info;
@@ -105,6 +108,8 @@ int main(int argc, char** argv)
                     S, Nd, gx.size(), gy.size(), gz.size(), dem.X_vec.size(), dem.Y_vec.size(), Tx.size(), n1, n2);
         if (check_only) return 0;
         if (S < 2 || Nd < 1) { std::fprintf(stderr, "[ERROR] rootData needs S >= 2 and >= 1 pulse\n"); return 1; }
+        const double sec_read = since(t_read);
+        const auto t_load = clk::now();
 
         const size_t n = Tx.size();
         std::vector<double> tx(3 * n), rx(3 * n);
@@ -118,23 +123,29 @@ int main(int argc, char** argv)
         // rootData flat index r + p*S is already pulse-major, range fastest
         int st = bsar_dataset_load_rootdata(ctx, ds, reinterpret_cast<const double*>(root.data()), (int)S, (int)Nd);
         if (st != BSAR_OK) { bsar_dataset_destroy(ds); fa::check(st, ctx, "rootData upload"); }
+        fa::check(bsar_ctx_synchronize(ctx), ctx, "rootData upload");
+        const double sec_load = since(t_load);
         const size_t nx = gx.size(), ny = gy.size(), nz = gz.size();
         std::vector<std::complex<double>> img(nx * ny * nz);
         bsar_report rep{};
-        const auto t0 = std::chrono::steady_clock::now();
+        const auto t0 = clk::now();
         st = bsar_backproject(ctx, ds, gx.data(), (int)nx, gy.data(), (int)ny, gz.data(), (int)nz, precision,
                               reinterpret_cast<double*>(img.data()), &rep);
-        const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const double wall = since(t0);
         bsar_dataset_destroy(ds);
         fa::check(st, ctx, "backproject");
         std::printf("[PERFORMANCE] back-projection: %.3f s device, %.3f s wall, %.3e voxel-pulse/s (%s)\n",
                     rep.seconds, wall, rep.voxel_pulses / (rep.seconds > 0 ? rep.seconds : 1),
                     precision == BSAR_MIXED ? "mixed" : "fp64");
+        const auto t_write = clk::now();
         const std::string out = dir + "/output_3D_matrix.bin";
         st = bsar_write_image(out.c_str(), reinterpret_cast<const double*>(img.data()), (int)nx, (int)ny, (int)nz,
                               gx.data(), gy.data(), gz.data());
         fa::check(st, nullptr, "export");
         std::printf("wrote %s\n", out.c_str());
+        std::printf("[PERFORMANCE] stages: read inputs %.3f s (rootData %.1f MB), upload + x%d spline %.3f s, "
+                    "back-projection %.3f s (image to host included), write image %.3f s (%.1f MB)\n",
+                    sec_read, S * Nd * 16 / 1e6, factor, sec_load, wall, since(t_write), nx * ny * nz * 16 / 1e6);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[ERROR] %s\n", e.what());
         return 1;
