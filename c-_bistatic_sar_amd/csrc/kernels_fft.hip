@@ -63,9 +63,9 @@ __device__ __forceinline__ typename C2<T>::type rot16(int r, int d, T sgn)
 
 // K consecutive radix-2 stages on the 2^K elements v[m] = x[e0 + m h] (h =
 // 2^lh, j = e0 mod h).  The stage at distance 2^t h pairs v[m], v[m + 2^t]
-// with twiddle exp(sgn i pi (j + (m mod 2^t) h) / (2^t h)) = W^(2^(K-1-t)) *
-// exp(sgn i pi (m mod 2^t) / 2^t), W = exp(sgn i pi j / (2^(K-1) h)) =
-// tw[j S / (2^K h)] (conjugated for the inverse), tw[k] = exp(-2 pi i k / S).
+// with twiddle exp(sgn i pi (j + (m mod 2^t) h) / (2^t h)) = W_t *
+// exp(sgn i pi (m mod 2^t) / 2^t), W_t = exp(sgn i pi j / (2^t h)) =
+// tw[j S / (2^(t+1) h)] (conjugated for the inverse), tw[k] = exp(-2 pi i k / S).
 // kDif: decimation in frequency (distances descending, twiddle after the
 // difference: natural order in, bit-reversed out); else decimation in time
 // (distances ascending, twiddle on the second input: bit-reversed in,
@@ -83,11 +83,13 @@ __device__ __forceinline__ void butterflies(typename C2<T>::type* v, int j, int 
 #pragma unroll
         for (int t = 0; t < K; ++t) wt[t] = V{(T)1, (T)0};
     } else {
-        V w = tw[(int64_t)j << (logS - K - lh)];
-        if (kInv) w.y = -w.y;
-        wt[K - 1] = w;
+        // W_t = exp(sgn i pi j / (2^t h)) = tw[j S / (2^(t+1) h)], read from the table
 #pragma unroll
-        for (int t = K - 2; t >= 0; --t) wt[t] = cmul(wt[t + 1], wt[t + 1]);
+        for (int t = 0; t < K; ++t) {
+            V w = tw[(int64_t)j << (logS - lh - 1 - t)];
+            if (kInv) w.y = -w.y;
+            wt[t] = w;
+        }
     }
 #pragma unroll
     for (int tt = 0; tt < K; ++tt) {
