@@ -279,7 +279,9 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     // Columns whose lower bound can reach M (bit jj <-> column center_out(jj),
     // fixed for the level: M only decreases, so later rows may estimate a
     // column the bound would now drop) and their FP32 terms: Z element
-    // offsets of the rows j0 and j0 + dj, weight, squared y offsets.
+    // offsets of the rows j0 and j0 + dj, weight, squared y offsets.  A
+    // dropped column keeps sy1 = +inf, which makes its estimate +inf (or NaN),
+    // so the screening loop needs no per-candidate mask test.
     int o0[G], o1[G];
     float wyf[G], sy1[G], sy2[G];
     unsigned colmask = 0u;
@@ -289,7 +291,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         o0[j] = 0;
         o1[j] = 0;
         wyf[j] = 0.0f;
-        sy1[j] = 0.0f;
+        sy1[j] = __builtin_inff();
         sy2[j] = 0.0f;
         const double pya = __builtin_fma((double)j, stepY, minY);
         const double a = fmax(fabs(pya - L.y1) - epsy, 0.0), b = fmax(fabs(pya - L.y2) - epsy, 0.0);
@@ -361,13 +363,14 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
                 const float A = __builtin_fmaf(e1, e1, dxaf + sy1[j]);
                 const float B = __builtin_fmaf(e2, e2, dxbf + sy2[j]);
                 const float os = __builtin_fmaf(L.n2f, sqrt_f32(B), L.n1f * sqrt_f32(A));
-                const bool ok = (colok >> jj) & 1u;
-                // Mf = min(Mf, (est + D32)(1 + 3 kRho32)); NaN estimates leave it
-                const float t = __builtin_fmaf(ok ? os : __builtin_inff(), k3, Dk3);
-                Mf = t < Mf ? t : Mf;
+                // Mf = min(Mf, (est + D32)(1 + 3 kRho32)); NaN and +inf (dropped
+                // column) estimates leave it; the probe's estimate bounds its
+                // exact value, which M already includes
+                Mf = __builtin_fminf(__builtin_fmaf(os, k3, Dk3), Mf);
                 const bool keep = !(os > __builtin_fmaf(Mf, k2, D2));
-                pass |= (ok & keep) ? (1u << jj) : 0u;
+                pass |= keep ? (1u << jj) : 0u;
             }
+            pass &= colok;
             M = fmin(M, (double)Mf);
         }
         // exact evaluation of the survivors
