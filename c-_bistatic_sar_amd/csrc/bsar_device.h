@@ -107,9 +107,9 @@ __device__ __forceinline__ double dem_interp(const DemDev& d, double x, double y
 //     the centre), and ties between exact values resolve to the lowest
 //     reference index i*G + j -- the first one the sequential scan meets.
 //
-// Error bounds.  Lower bounds: v_rsq_f64 is within 5.3e-8 relative of
-// 1/sqrt (measured on gfx950); kDefl deflates them by more than that and
-// their few roundings.  Estimates: with u = 2^-24, the FP32 inputs (Z, the
+// Error bounds.  Lower bounds (lower_bound32): FP32 with v_rsq_f32 (about 1
+// ulp), at most 4.8e-7 relative in all; kDefl deflates them by 8e-6, more
+// than an order of magnitude beyond that.  Estimates: with u = 2^-24, the FP32 inputs (Z, the
 // bilinear weights, z1, z2, the squared horizontal offsets, n1, n2: one
 // rounding each), the bilinear blend (~30 u |Z|max absolute), the sums and
 // v_sqrt_f32 (1 ulp) give
@@ -127,6 +127,19 @@ constexpr double kDefl = 1.0 - 8e-6;
 
 __device__ __forceinline__ double sqrt_est(double a) { return a * __builtin_amdgcn_rsq(a); }
 __device__ __forceinline__ float sqrt_f32(float a) { return __builtin_amdgcn_sqrtf(a); }
+
+// Deflated lower bound n1 sqrt(a^2 + g1sq) + n2 sqrt(b^2 + g2sq) in FP32 for
+// a, b >= 0: the roundings of a, b, g1sq, g2sq, n1, n2 to FP32, the fma and
+// v_rsq_f32 (1 ulp) keep the relative error below 8 * 2^-24 = 4.8e-7, far
+// inside the 8e-6 of kDefl, so the result stays below the exact bound.  A
+// zero argument gives 0 * inf = NaN, which never prunes.
+__device__ __forceinline__ double lower_bound32(double a, double b, float g1f, float g2f, float n1f, float n2f)
+{
+    const float af = (float)a, bf = (float)b;
+    const float s1 = __builtin_fmaf(af, af, g1f), s2 = __builtin_fmaf(bf, bf, g2f);
+    const float lb = __builtin_fmaf(n2f, s2 * __builtin_amdgcn_rsqf(s2), n1f * (s1 * __builtin_amdgcn_rsqf(s1)));
+    return (double)(lb * (float)kDefl);
+}
 
 struct Leg {
     double x1, y1, z1, x2, y2, z2, n1, n2;
@@ -249,7 +262,6 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     const bool empty = (d.flags & DEM_EMPTY) != 0;
     const bool ysorted = (d.flags & DEM_Y_SORTED) != 0;
     const bool xsorted = (d.flags & DEM_X_SORTED) != 0;
-    const double n1 = L.n1, n2 = L.n2;
     const double stepX = spanX / (G - 1), stepY = spanY / (G - 1);
     const double epsx = 1e-12 * (fabs(minX) + fabs(spanX) + 1.0);
     const double epsy = 1e-12 * (fabs(minY) + fabs(spanY) + 1.0);
@@ -257,6 +269,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     // range inside this level's window once the window is small
     double g1sq = L.g1sq, g2sq = L.g2sq;
     if (G == 11 && !empty && xsorted && ysorted) local_gaps(d, L.z1, L.z2, minX, maxX, minY, maxY, g1sq, g2sq);
+    const float g1f = (float)g1sq, g2f = (float)g2sq;
 
     // Probe: the centre candidate (the previous level's best) evaluated
     // exactly first, so the bounds below start from a tight value.
@@ -295,8 +308,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         sy2[j] = 0.0f;
         const double pya = __builtin_fma((double)j, stepY, minY);
         const double a = fmax(fabs(pya - L.y1) - epsy, 0.0), b = fmax(fabs(pya - L.y2) - epsy, 0.0);
-        const double lb = (n1 * sqrt_est(__builtin_fma(a, a, g1sq)) + n2 * sqrt_est(__builtin_fma(b, b, g2sq))) * kDefl;
-        if (lb > M) continue;
+        if (lower_bound32(a, b, g1f, g2f, L.n1f, L.n2f) > M) continue;
         colmask |= 1u << jj;
         const double py = minY + j * spanY / (G - 1);
         int j0 = 0, dj = 0;
@@ -322,9 +334,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
             const double pxa = __builtin_fma((double)i, stepX, minX);
             const double a1 = fmax(fabs(pxa - L.x1) - epsx, 0.0);
             const double a2 = fmax(fabs(pxa - L.x2) - epsx, 0.0);
-            const double lbr = (n1 * sqrt_est(__builtin_fma(a1, a1, g1sq)) +
-                                n2 * sqrt_est(__builtin_fma(a2, a2, g2sq))) * kDefl;
-            if (lbr > M) continue;
+            if (lower_bound32(a1, a2, g1f, g2f, L.n1f, L.n2f) > M) continue;
         }
         // the probe (i, j) = (C, C) is bit 0 of row C
         const unsigned colok = (i == C) ? (colmask & ~1u) : colmask;
