@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace bsar {
 
 enum : int { DEM_EMPTY = 1, DEM_X_SORTED = 2, DEM_Y_SORTED = 4 };
@@ -251,7 +253,7 @@ __device__ __forceinline__ double exact_opt(const DemDev& d, const Leg& L, bool 
     return L.n1 * d1 + L.n2 * d2;
 }
 
-template <int G>
+template <int G, bool kSplit>
 __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, double R, double& bestX, double& bestY,
                                              double& minOpt)
 {
@@ -289,37 +291,40 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         M = L.screen ? fmin(minOpt, lvMin) : __builtin_inf();
     }
 
-    // Columns whose lower bound can reach M (bit jj <-> column center_out(jj),
-    // fixed for the level: M only decreases, so later rows may estimate a
-    // column the bound would now drop) and their FP32 terms: Z element
-    // offsets of the rows j0 and j0 + dj, weight, squared y offsets.  A
-    // dropped column keeps sy1 = +inf, which makes its estimate +inf (or NaN),
-    // so the screening loop needs no per-candidate mask test.
-    int o0[G], o1[G];
-    float wyf[G], sy1[G], sy2[G];
+    // The columns jj = JB .. JB + NJ - 1 (centre-out order) against every row.
+    // Columns whose lower bound can reach M (bit t <-> column
+    // center_out(JB + t), fixed for the part: M only decreases, so later rows
+    // may estimate a column the bound would now drop) and their FP32 terms: Z
+    // element offsets of the rows j0 and j0 + dj, weight, squared y offsets.
+    // A dropped column keeps sy1 = +inf, which makes its estimate +inf (or
+    // NaN), so the screening loop needs no per-candidate mask test.
+    auto part = [&](auto jb, auto nj) {
+    constexpr int JB = decltype(jb)::value, NJ = decltype(nj)::value;
+    int o0[NJ], o1[NJ];
+    float wyf[NJ], sy1[NJ], sy2[NJ];
     unsigned colmask = 0u;
 #pragma unroll
-    for (int jj = 0; jj < G; ++jj) {
-        const int j = center_out(jj, C);
-        o0[j] = 0;
-        o1[j] = 0;
-        wyf[j] = 0.0f;
-        sy1[j] = __builtin_inff();
-        sy2[j] = 0.0f;
+    for (int t = 0; t < NJ; ++t) {
+        const int j = center_out(JB + t, C);
+        o0[t] = 0;
+        o1[t] = 0;
+        wyf[t] = 0.0f;
+        sy1[t] = __builtin_inff();
+        sy2[t] = 0.0f;
         const double pya = __builtin_fma((double)j, stepY, minY);
         const double a = fmax(fabs(pya - L.y1) - epsy, 0.0), b = fmax(fabs(pya - L.y2) - epsy, 0.0);
         if (lower_bound32(a, b, g1f, g2f, L.n1f, L.n2f) > M) continue;
-        colmask |= 1u << jj;
+        colmask |= 1u << t;
         const double py = minY + j * spanY / (G - 1);
         int j0 = 0, dj = 0;
         double w = 0.0;
         if (!empty) axis_lookup(d.Y, d.ny, d.y0, d.ihy, ysorted, py, j0, dj, w);
-        o0[j] = j0 * d.zs;
-        o1[j] = (j0 + dj) * d.zs;
-        wyf[j] = (float)w;
+        o0[t] = j0 * d.zs;
+        o1[t] = (j0 + dj) * d.zs;
+        wyf[t] = (float)w;
         const double dy1 = py - L.y1, dy2 = L.y2 - py;
-        sy1[j] = (float)(dy1 * dy1);
-        sy2[j] = (float)(dy2 * dy2);
+        sy1[t] = (float)(dy1 * dy1);
+        sy2[t] = (float)(dy2 * dy2);
     }
     // FP32 screening constants (rounded to FP32: well inside the 2 kRho32 and
     // D32 slack of the bounds)
@@ -336,8 +341,8 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
             const double a2 = fmax(fabs(pxa - L.x2) - epsx, 0.0);
             if (lower_bound32(a1, a2, g1f, g2f, L.n1f, L.n2f) > M) continue;
         }
-        // the probe (i, j) = (C, C) is bit 0 of row C
-        const unsigned colok = (i == C) ? (colmask & ~1u) : colmask;
+        // the probe (i, j) = (C, C) is bit 0 of row C in the first part
+        const unsigned colok = (JB == 0 && i == C) ? (colmask & ~1u) : colmask;
         if (!colok) continue;
         const double px = minX + i * spanX / (G - 1);
         int i0 = 0, di = 0;
@@ -351,7 +356,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         // loads of a row issue together.  When the x interval was not found
         // (di = 0) the second node gets weight 0: a non-finite neighbour then
         // makes the estimate NaN, which only sends the candidate to the exact
-        // evaluation.  Bit jj of pass: candidate (i, center_out(jj)) may
+        // evaluation.  Bit t of pass: candidate (i, center_out(JB + t)) may
         // still be the minimum.
         unsigned pass = colok;
         if (L.screen) {
@@ -360,39 +365,48 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
             const float wxf = di ? (float)wx : 0.0f, dxaf = (float)dxa, dxbf = (float)dxb;
             float Mf = (float)(M * (1.0 + 0x1p-22));   // >= M: an upper bound of the minimum
 #pragma unroll
-            for (int jj = 0; jj < G; ++jj) {
-                const int j = center_out(jj, C);
-                const float* p0 = zr + o0[j];
-                const float* p1 = zr + o1[j];
+            for (int t = 0; t < NJ; ++t) {
+                const float* p0 = zr + o0[t];
+                const float* p1 = zr + o1[t];
                 const float z00 = p0[0], z01 = p0[1];
                 const float z10 = p1[0], z11 = p1[1];
                 const float za = __builtin_fmaf(z01 - z00, wxf, z00);
                 const float zb = __builtin_fmaf(z11 - z10, wxf, z10);
-                const float ze = __builtin_fmaf(zb - za, wyf[j], za);
+                const float ze = __builtin_fmaf(zb - za, wyf[t], za);
                 const float e1 = ze - L.z1f, e2 = L.z2f - ze;
-                const float A = __builtin_fmaf(e1, e1, dxaf + sy1[j]);
-                const float B = __builtin_fmaf(e2, e2, dxbf + sy2[j]);
+                const float A = __builtin_fmaf(e1, e1, dxaf + sy1[t]);
+                const float B = __builtin_fmaf(e2, e2, dxbf + sy2[t]);
                 const float os = __builtin_fmaf(L.n2f, sqrt_f32(B), L.n1f * sqrt_f32(A));
                 // Mf = min(Mf, (est + D32)(1 + 3 kRho32)); NaN and +inf (dropped
                 // column) estimates leave it; the probe's estimate bounds its
                 // exact value, which M already includes
                 Mf = __builtin_fminf(__builtin_fmaf(os, k3, Dk3), Mf);
                 const bool keep = !(os > __builtin_fmaf(Mf, k2, D2));
-                pass |= keep ? (1u << jj) : 0u;
+                pass |= keep ? (1u << t) : 0u;
             }
             pass &= colok;
             M = fmin(M, (double)Mf);
         }
         // exact evaluation of the survivors
         while (pass) {
-            const int jj = __builtin_ctz(pass);
+            const int t = __builtin_ctz(pass);
             pass &= pass - 1u;
-            const int j = center_out(jj, C);
+            const int j = center_out(JB + t, C);
             const double opt = exact_opt(d, L, empty, ysorted, i0, di, wx, dxa, dxb, minY + j * spanY / (G - 1));
             const int kr = i * G + j;
             if (opt < lvMin || (opt == lvMin && kr < lvK)) { lvMin = opt; lvK = kr; }
             if (L.screen) M = fmin(M, lvMin);
         }
+    }
+    };
+    // kSplit: a 15-wide level runs its columns in two parts (8 + 7), which
+    // keeps the per-column arrays at the size of the 11-wide levels' (for
+    // kernels whose register budget would otherwise spill them)
+    if (kSplit && G == 15) {
+        part(std::integral_constant<int, 0>{}, std::integral_constant<int, 8>{});
+        part(std::integral_constant<int, 8>{}, std::integral_constant<int, G - 8>{});
+    } else {
+        part(std::integral_constant<int, 0>{}, std::integral_constant<int, G>{});
     }
     if (lvMin < minOpt) {
         minOpt = lvMin;
@@ -403,6 +417,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
 }
 
 // calculateRefractionPoint, raw_data_processing.cpp:455-572.
+template <bool kSplit = false>
 __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double y1, double z1,
                                              double x2, double y2, double z2, double n1, double n2,
                                              double& qx, double& qy, double& qz)
@@ -413,11 +428,11 @@ __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double 
     double R = std_max(__builtin_sqrt(ex * ex + ey * ey), 500.0);    // :483-484
     double minOpt = 1e9;                                             // :497
     const Leg L = make_leg(d, x1, y1, z1, x2, y2, z2, n1, n2);
-    fermat_level<15>(d, L, R, bestX, bestY, minOpt);
+    fermat_level<15, kSplit>(d, L, R, bestX, bestY, minOpt);
     R /= 5.0;                                                        // :551
 #pragma unroll 1
     for (int level = 1; level < 5; ++level) {
-        fermat_level<11>(d, L, R, bestX, bestY, minOpt);
+        fermat_level<11, kSplit>(d, L, R, bestX, bestY, minOpt);
         R /= 3.0;
     }
     qx = bestX;                                                      // :555-557

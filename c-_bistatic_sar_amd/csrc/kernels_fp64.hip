@@ -117,7 +117,7 @@ __global__ __launch_bounds__(dem_block(kMode, 4), dem_waves(kMode, 4)) void k_bp
     const int p0 = gbeg + a.woff;
     const int p1 = min(p0 + a.wlen, gend);
     if (p0 >= p1) return;
-    bp_fp64_thread(d, a, g, a.vox0 + vl, p0, p1);
+    bp_fp64_thread<kMode == DEM_LDS>(d, a, g, a.vox0 + vl, p0, p1);
 }
 
 // out[v] = sum over groups in order; out is [nvox] complex (x-fastest voxels).
