@@ -67,21 +67,33 @@ int upload_dem(bsar_ctx* ctx, const bsar_dem* dem, double** dX, double** dY, dou
     // refraction search's screening estimates read only the FP32 copy)
     std::vector<float> Zf(Z.size());
     for (size_t k = 0; k < Z.size(); ++k) Zf[k] = (float)Z[k];
+    // ... and by (height, x difference to the next node) pairs in FP32, one
+    // rounding each, for the screening of DEMs staged whole in LDS
+    std::vector<float> Zp(2 * Z.size());
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < zs; ++i) {
+            const size_t k = (size_t)j * zs + i;
+            Zp[2 * k] = (float)Z[k];
+            Zp[2 * k + 1] = i + 1 < zs ? (float)(Z[k + 1] - Z[k]) : 0.0f;
+        }
     size_t bx = X.size() * sizeof(double), by = Y.size() * sizeof(double), bz = Z.size() * sizeof(double);
-    size_t bzf = Zf.size() * sizeof(float);
+    size_t bzf = (Zf.size() * sizeof(float) + 7) / 8 * 8, bzp = Zp.size() * sizeof(float);
     BSAR_HIP(ctx, hipMalloc((void**)dX, bx));
     BSAR_HIP(ctx, hipMalloc((void**)dY, by));
-    BSAR_HIP(ctx, hipMalloc((void**)dZ, bz + bzf));
+    BSAR_HIP(ctx, hipMalloc((void**)dZ, bz + bzf + bzp));
     BSAR_HIP(ctx, hipMemcpyAsync(*dX, X.data(), bx, hipMemcpyHostToDevice, ctx->stream));
     BSAR_HIP(ctx, hipMemcpyAsync(*dY, Y.data(), by, hipMemcpyHostToDevice, ctx->stream));
     BSAR_HIP(ctx, hipMemcpyAsync(*dZ, Z.data(), bz, hipMemcpyHostToDevice, ctx->stream));
-    BSAR_HIP(ctx, hipMemcpyAsync((char*)*dZ + bz, Zf.data(), bzf, hipMemcpyHostToDevice, ctx->stream));
+    BSAR_HIP(ctx, hipMemcpyAsync((char*)*dZ + bz, Zf.data(), Zf.size() * sizeof(float), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    BSAR_HIP(ctx, hipMemcpyAsync((char*)*dZ + bz + bzf, Zp.data(), bzp, hipMemcpyHostToDevice, ctx->stream));
     BSAR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     DemDev d{};
     d.X = *dX;
     d.Y = *dY;
     d.Z = *dZ;
     d.Zf = reinterpret_cast<const float*>((char*)*dZ + bz);
+    d.Zp = reinterpret_cast<const float2*>((char*)*dZ + bz + bzf);
     d.nx = dem->nx;
     d.ny = dem->ny;
     d.zs = zs;

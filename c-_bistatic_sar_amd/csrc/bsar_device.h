@@ -25,6 +25,7 @@ struct DemDev {
     const double* Y;
     const double* Z;
     const float* Zf;           // Z rounded to FP32, same layout (screening estimates only)
+    const float2* Zp;          // (Z, Z[.][i+1] - Z[.][i]) in FP32 per node, same layout (screening, DEM in LDS)
     int nx, ny, zs, flags;
     double x0, ihx, y0, ihy;   // first value and inverse mean spacing (index guess only)
     double zlo, zhi;           // height range of the stored Z (0, 0 when empty)
@@ -253,7 +254,7 @@ __device__ __forceinline__ double exact_opt(const DemDev& d, const Leg& L, bool 
     return L.n1 * d1 + L.n2 * d2;
 }
 
-template <int G, bool kSplit>
+template <int G, bool kSplit, bool kPairs>
 __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, double R, double& bestX, double& bestY,
                                              double& minOpt)
 {
@@ -362,16 +363,25 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         if (L.screen) {
             pass = 0u;
             const float* zr = d.Zf + i0;
+            const float2* zp = d.Zp + i0;
             const float wxf = di ? (float)wx : 0.0f, dxaf = (float)dxa, dxbf = (float)dxb;
             float Mf = (float)(M * (1.0 + 0x1p-22));   // >= M: an upper bound of the minimum
 #pragma unroll
             for (int t = 0; t < NJ; ++t) {
-                const float* p0 = zr + o0[t];
-                const float* p1 = zr + o1[t];
-                const float z00 = p0[0], z01 = p0[1];
-                const float z10 = p1[0], z11 = p1[1];
-                const float za = __builtin_fmaf(z01 - z00, wxf, z00);
-                const float zb = __builtin_fmaf(z11 - z10, wxf, z10);
+                float za, zb;
+                if (kPairs) {
+                    // (z, dz/dx) pairs: one 8-byte LDS load per row of the cell
+                    const float2 a0 = zp[o0[t]], a1 = zp[o1[t]];
+                    za = __builtin_fmaf(a0.y, wxf, a0.x);
+                    zb = __builtin_fmaf(a1.y, wxf, a1.x);
+                } else {
+                    const float* p0 = zr + o0[t];
+                    const float* p1 = zr + o1[t];
+                    const float z00 = p0[0], z01 = p0[1];
+                    const float z10 = p1[0], z11 = p1[1];
+                    za = __builtin_fmaf(z01 - z00, wxf, z00);
+                    zb = __builtin_fmaf(z11 - z10, wxf, z10);
+                }
                 const float ze = __builtin_fmaf(zb - za, wyf[t], za);
                 const float e1 = ze - L.z1f, e2 = L.z2f - ze;
                 const float A = __builtin_fmaf(e1, e1, dxaf + sy1[t]);
@@ -417,7 +427,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
 }
 
 // calculateRefractionPoint, raw_data_processing.cpp:455-572.
-template <bool kSplit = false>
+template <bool kSplit = false, bool kPairs = false>
 __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double y1, double z1,
                                              double x2, double y2, double z2, double n1, double n2,
                                              double& qx, double& qy, double& qz)
@@ -428,11 +438,11 @@ __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double 
     double R = std_max(__builtin_sqrt(ex * ex + ey * ey), 500.0);    // :483-484
     double minOpt = 1e9;                                             // :497
     const Leg L = make_leg(d, x1, y1, z1, x2, y2, z2, n1, n2);
-    fermat_level<15, kSplit>(d, L, R, bestX, bestY, minOpt);
+    fermat_level<15, kSplit, kPairs>(d, L, R, bestX, bestY, minOpt);
     R /= 5.0;                                                        // :551
 #pragma unroll 1
     for (int level = 1; level < 5; ++level) {
-        fermat_level<11, kSplit>(d, L, R, bestX, bestY, minOpt);
+        fermat_level<11, kSplit, kPairs>(d, L, R, bestX, bestY, minOpt);
         R /= 3.0;
     }
     qx = bestX;                                                      // :555-557
@@ -440,10 +450,10 @@ __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double 
     qz = dem_interp(d, bestX, bestY);
 }
 
-// 8-byte LDS slots of a staged DEM: X, Y, Z (FP64), then Zf (FP32).
+// 8-byte LDS slots of a staged DEM: X, Y, Z (FP64), then Zp (FP32 pairs).
 __host__ __device__ __forceinline__ int dem_lds_slots(const DemDev& g)
 {
-    return g.nx + g.ny + g.ny * g.zs + (g.ny * g.zs + 1) / 2;
+    return g.nx + g.ny + 2 * g.ny * g.zs;
 }
 
 // View of a DEM staged at lds (no copy).
@@ -453,7 +463,7 @@ __device__ __forceinline__ DemDev dem_lds_view(const DemDev& g, double* lds)
     d.X = lds;
     d.Y = lds + g.nx;
     d.Z = lds + g.nx + g.ny;
-    d.Zf = reinterpret_cast<const float*>(lds + g.nx + g.ny + g.ny * g.zs);
+    d.Zp = reinterpret_cast<const float2*>(lds + g.nx + g.ny + g.ny * g.zs);
     return d;
 }
 
@@ -491,8 +501,8 @@ __device__ __forceinline__ DemDev dem_to_lds(const DemDev& g, double* lds)
         else v = g.Z[k - g.nx - g.ny];
         lds[k] = v;
     }
-    float* zf = reinterpret_cast<float*>(lds + total);
-    for (int k = threadIdx.x; k < nz; k += blockDim.x) zf[k] = g.Zf[k];
+    float2* zp = reinterpret_cast<float2*>(lds + total);
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) zp[k] = g.Zp[k];
     return dem_lds_view(g, lds);
 }
 

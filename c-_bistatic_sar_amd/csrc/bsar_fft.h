@@ -11,12 +11,15 @@ namespace bsar {
 constexpr size_t kFftLdsMax = 128 * 1024;
 
 // data[c] <- IFFT(FFT(data[c]) .* H) for ncols contiguous columns of length
-// S (power of two).  tmp: scratch of fft_filter_tmp_elems(S, ncols, bytes
-// per complex) complex elements -- ncols*S for the global path, S (the
-// spectrum in bit-reversed order) for the LDS path.
+// S (power of two).  Columns within the LDS budget run in LDS; columns of
+// twice that as two half-length LDS filters between a global radix-2 split
+// and merge; longer ones as global radix-2 passes.  tmp: scratch of
+// fft_filter_tmp_elems(S, ncols, bytes per complex) complex elements -- S
+// (the spectrum in bit-reversed order) for the LDS paths, ncols*S for the
+// global path.
 inline size_t fft_filter_tmp_elems(size_t S, size_t ncols, size_t elem_bytes)
 {
-    return S * elem_bytes > kFftLdsMax ? S * ncols : S;
+    return (S / 2) * elem_bytes > kFftLdsMax ? S * ncols : S;
 }
 hipError_t fft_filter(double2* data, double2* tmp, const double2* H, int S, int64_t ncols, hipStream_t st);
 hipError_t fft_filter(float2* data, float2* tmp, const float2* H, int S, int64_t ncols, hipStream_t st);

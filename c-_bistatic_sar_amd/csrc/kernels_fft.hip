@@ -9,8 +9,10 @@
 // workgroup: the column is staged in LDS, transformed in place (radix-2 DIT
 // forward from bit-reversed load, DIF inverse to bit-reversed store, three
 // stages per LDS round trip, twiddles from a precomputed table), so HBM sees
-// one read and one write per sample.  Columns larger than the LDS budget
-// run as log2(S) global Stockham passes per direction.
+// one read and one write per sample.  Columns of twice the LDS budget are
+// split by one global radix-2 stage into two half-length columns filtered in
+// LDS and merged by one global radix-2 stage (three passes over the data);
+// longer ones run as log2(S) global Stockham passes per direction.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -201,7 +203,8 @@ template <typename T, int KP, int Kb>
 __global__ __launch_bounds__(fft_max_threads<T>()) void k_fft_filter_lds(const typename C2<T>::type* __restrict__ in,
                                                          typename C2<T>::type* __restrict__ out,
                                                          const typename C2<T>::type* __restrict__ Hbr,
-                                                         const typename C2<T>::type* __restrict__ tw, int logS)
+                                                         const typename C2<T>::type* __restrict__ tw, int logS,
+                                                         int halves)
 {
     using V = typename C2<T>::type;
     constexpr int P = 1 << KP;
@@ -212,6 +215,8 @@ __global__ __launch_bounds__(fft_max_threads<T>()) void k_fft_filter_lds(const t
     const int t = threadIdx.x;
     const int top = logS - KP;              // register stages have distances 2^top .. S/2
     const int64_t col = blockIdx.x;
+    // halves: columns alternate between the two spectra Hbr[0, S) and Hbr[S, 2S)
+    const V* Hc = Hbr + (halves ? (int64_t)(col & 1) * S : 0);
     V v[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) v[i] = in[col * S + t + i * nt];
@@ -219,12 +224,12 @@ __global__ __launch_bounds__(fft_max_threads<T>()) void k_fft_filter_lds(const t
     if (top == 0) {
         // S = 2^KP: the registers hold the whole spectrum (bit-reversed)
 #pragma unroll
-        for (int i = 0; i < P; ++i) v[i] = cmul(v[i], Hbr[i]);
+        for (int i = 0; i < P; ++i) v[i] = cmul(v[i], Hc[i]);
     } else {
 #pragma unroll
         for (int i = 0; i < P; ++i) x[lds_pad(t + i * nt)] = v[i];
         __syncthreads();
-        lds_filter<T, Kb>(x, top, logS, Hbr, tw);
+        lds_filter<T, Kb>(x, top, logS, Hc, tw);
 #pragma unroll
         for (int i = 0; i < P; ++i) v[i] = x[lds_pad(t + i * nt)];
     }
@@ -232,6 +237,57 @@ __global__ __launch_bounds__(fft_max_threads<T>()) void k_fft_filter_lds(const t
     const T inv = (T)1 / (T)S;
 #pragma unroll
     for (int i = 0; i < P; ++i) out[col * S + t + i * nt] = V{v[i].x * inv, v[i].y * inv};
+}
+
+// Columns of length S = 2n above the LDS budget run as two independent
+// length-n filters: the first radix-2 DIF stage splits X = DFT_S(x) into its
+// even bins (DFT_n of x[k] + x[k+n]) and odd bins (DFT_n of (x[k] - x[k+n])
+// W^k, W = exp(-2 pi i / S)); each half is filtered by its half of the
+// spectrum in the LDS kernel (which scales by 1/n); the last radix-2 DIT
+// stage of the inverse recombines them, y[k] = (a[k] + b[k] W^-k) / 2 and
+// y[k+n] = (a[k] - b[k] W^-k) / 2.  tw is the table of length-S twiddles.
+template <typename T>
+__global__ void k_split_dif(typename C2<T>::type* __restrict__ x, const typename C2<T>::type* __restrict__ tw,
+                            int n, int64_t ncols)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= ncols * n) return;
+    const int64_t col = t / n;
+    const int k = (int)(t - col * n);
+    auto* c = x + col * 2 * n;
+    const auto u = c[k], v = c[k + n];
+    c[k] = cadd(u, v);
+    c[k + n] = cmul(csub(u, v), tw[k]);
+}
+
+template <typename T>
+__global__ void k_split_dit(typename C2<T>::type* __restrict__ x, const typename C2<T>::type* __restrict__ tw,
+                            int n, int64_t ncols)
+{
+    using V = typename C2<T>::type;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= ncols * n) return;
+    const int64_t col = t / n;
+    const int k = (int)(t - col * n);
+    V* c = x + col * 2 * n;
+    const V w = tw[k];
+    const V a = c[k], b = cmul(c[k + n], V{w.x, -w.y});
+    const T h = (T)0.5;
+    c[k] = V{(a.x + b.x) * h, (a.y + b.y) * h};
+    c[k + n] = V{(a.x - b.x) * h, (a.y - b.y) * h};
+}
+
+// Hbr2 = [H[2 bitrev(p)] | H[2 bitrev(p) + 1]], p < n = 2^logn: the two
+// half-spectra, each in the bit-reversed order of a length-n transform.
+template <typename T>
+__global__ void k_split_spectrum(const typename C2<T>::type* __restrict__ H, typename C2<T>::type* __restrict__ Hbr2,
+                                 int logn)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = 1 << logn;
+    if (k >= 2 * n) return;
+    const int h = k >> logn, p = k & (n - 1);
+    Hbr2[k] = H[2 * (int)bitrev((uint32_t)p, logn) + h];
 }
 
 // Hbr[k] = H[bitrev(k)].
@@ -354,7 +410,8 @@ static size_t lds_bytes(int S) { return (size_t)(S + S / 32 + 1) * sizeof(typena
 
 template <typename T, int KP, int Kb>
 static hipError_t launch_lds_filter(typename C2<T>::type* data, const typename C2<T>::type* Hbr,
-                                    const typename C2<T>::type* tw, int logS, int64_t ncols, hipStream_t st)
+                                    const typename C2<T>::type* tw, int logS, int64_t ncols, hipStream_t st,
+                                    int halves = 0)
 {
     const int S = 1 << logS;
     if ((S >> KP) > fft_max_threads<T>()) return hipErrorInvalidValue;
@@ -363,7 +420,7 @@ static hipError_t launch_lds_filter(typename C2<T>::type* data, const typename C
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_fft_filter_lds<T, KP, Kb>), dim3((unsigned)ncols), dim3(S >> KP), lb, st, data, data, Hbr,
-                       tw, logS);
+                       tw, logS, halves);
     return hipGetLastError();
 }
 
@@ -401,6 +458,23 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
     using V = typename C2<T>::type;
     if (S < 2 || (S & (S - 1)) || ncols <= 0) return hipErrorInvalidValue;
     const int logS = ilog2(S);
+    // S <= 16: the register pass holds the column; else 16 samples per thread
+    // and the innermost (logS - 4) mod 4 (or 4) stages fused
+    auto lds = [&](int lg, const V* Hbr, const V* tw, int64_t nc, int halves) -> hipError_t {
+        switch (lg) {
+        case 1: return launch_lds_filter<T, 1, 1>(data, Hbr, tw, lg, nc, st, halves);
+        case 2: return launch_lds_filter<T, 2, 1>(data, Hbr, tw, lg, nc, st, halves);
+        case 3: return launch_lds_filter<T, 3, 1>(data, Hbr, tw, lg, nc, st, halves);
+        case 4: return launch_lds_filter<T, 4, 1>(data, Hbr, tw, lg, nc, st, halves);
+        default:
+            switch ((lg - 4) % 4) {
+            case 1: return launch_lds_filter<T, 4, 1>(data, Hbr, tw, lg, nc, st, halves);
+            case 2: return launch_lds_filter<T, 4, 2>(data, Hbr, tw, lg, nc, st, halves);
+            case 3: return launch_lds_filter<T, 4, 3>(data, Hbr, tw, lg, nc, st, halves);
+            default: return launch_lds_filter<T, 4, 4>(data, Hbr, tw, lg, nc, st, halves);
+            }
+        }
+    };
     if ((size_t)S * sizeof(V) <= kFftLdsMax) {
         // S / 2^KP <= 1024 threads: S <= 16384 (complex64) / 8192 (complex128)
         if (!tmp) return hipErrorInvalidValue;
@@ -409,21 +483,22 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_bitrev_copy<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        // S <= 16: the register pass holds the column; else 16 samples per
-        // thread and the innermost (logS - 4) mod 4 (or 4) stages fused
-        switch (logS) {
-        case 1: return launch_lds_filter<T, 1, 1>(data, tmp, tw, logS, ncols, st);
-        case 2: return launch_lds_filter<T, 2, 1>(data, tmp, tw, logS, ncols, st);
-        case 3: return launch_lds_filter<T, 3, 1>(data, tmp, tw, logS, ncols, st);
-        case 4: return launch_lds_filter<T, 4, 1>(data, tmp, tw, logS, ncols, st);
-        default:
-            switch ((logS - 4) % 4) {
-            case 1: return launch_lds_filter<T, 4, 1>(data, tmp, tw, logS, ncols, st);
-            case 2: return launch_lds_filter<T, 4, 2>(data, tmp, tw, logS, ncols, st);
-            case 3: return launch_lds_filter<T, 4, 3>(data, tmp, tw, logS, ncols, st);
-            default: return launch_lds_filter<T, 4, 4>(data, tmp, tw, logS, ncols, st);
-            }
-        }
+        return lds(logS, tmp, tw, ncols, 0);
+    }
+    if ((size_t)(S / 2) * sizeof(V) <= kFftLdsMax) {
+        // two half-length LDS filters between a DIF split and a DIT merge
+        if (!tmp) return hipErrorInvalidValue;
+        const int n = S / 2;
+        const V *tw = nullptr, *twn = nullptr;
+        hipError_t e = twiddle_table<T>(S, st, &tw);
+        if (e == hipSuccess) e = twiddle_table<T>(n, st, &twn);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_split_spectrum<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS - 1);
+        hipLaunchKernelGGL(k_split_dif<T>, dim3(nblk(ncols * n, 256)), dim3(256), 0, st, data, tw, n, ncols);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = lds(logS - 1, tmp, twn, 2 * ncols, 1)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_split_dit<T>, dim3(nblk(ncols * n, 256)), dim3(256), 0, st, data, tw, n, ncols);
+        return hipGetLastError();
     }
     V* res = nullptr;
     hipError_t e = fft_global<T>(data, tmp, S, ncols, -1, st, &res);

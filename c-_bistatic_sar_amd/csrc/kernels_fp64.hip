@@ -37,7 +37,7 @@ __global__ __launch_bounds__(dem_block(kMode, 3), dem_waves(kMode, 3)) void k_re
     if (k >= m) return;
     const double* s = legs + 6 * k;
     double qx, qy, qz;
-    fermat_solve(d, s[0], s[1], s[2], s[3], s[4], s[5], n1, n2, qx, qy, qz);
+    fermat_solve<false, kMode == DEM_LDS>(d, s[0], s[1], s[2], s[3], s[4], s[5], n1, n2, qx, qy, qz);
     out[3 * k] = qx;
     out[3 * k + 1] = qy;
     out[3 * k + 2] = qz;
@@ -62,9 +62,9 @@ __global__ __launch_bounds__(dem_block(kMode, 3), dem_waves(kMode, 3)) void k_sl
     const int64_t o = 3 * (v * tr.n + i);
     const double Px = pts[3 * v], Py = pts[3 * v + 1], Pz = pts[3 * v + 2];
     double qx, qy, qz;
-    fermat_solve(d, tr.tx[i], tr.ty[i], tr.tz[i], Px, Py, Pz, n1, n2, qx, qy, qz);
+    fermat_solve<false, kMode == DEM_LDS>(d, tr.tx[i], tr.ty[i], tr.tz[i], Px, Py, Pz, n1, n2, qx, qy, qz);
     ida[o] = qx; ida[o + 1] = qy; ida[o + 2] = qz;
-    fermat_solve(d, Px, Py, Pz, tr.rx[i], tr.ry[i], tr.rz[i], n2, n1, qx, qy, qz);
+    fermat_solve<false, kMode == DEM_LDS>(d, Px, Py, Pz, tr.rx[i], tr.ry[i], tr.rz[i], n2, n1, qx, qy, qz);
     volta[o] = qx; volta[o + 1] = qy; volta[o + 2] = qz;
 }
 
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(dem_block(kMode, 4), dem_waves(kMode, 4)) void k_bp
     const int p0 = gbeg + a.woff;
     const int p1 = min(p0 + a.wlen, gend);
     if (p0 >= p1) return;
-    bp_fp64_thread<kMode == DEM_LDS>(d, a, g, a.vox0 + vl, p0, p1);
+    bp_fp64_thread<kMode == DEM_LDS, kMode == DEM_LDS>(d, a, g, a.vox0 + vl, p0, p1);
 }
 
 // out[v] = sum over groups in order; out is [nvox] complex (x-fastest voxels).

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(dem_block(kMode, 4), dem_waves(kMode, 4)) void k_bp
     const int p1 = min(p0 + a.wlen, gend);
     if (p0 >= p1) return;
     unsigned exact = 0;
-    bp_mixed_thread<kMode == DEM_LDS>(d, a, m, g, a.vox0 + vl, p0, p1, exact);
+    bp_mixed_thread<kMode == DEM_LDS, kMode == DEM_LDS>(d, a, m, g, a.vox0 + vl, p0, p1, exact);
     if (exact && m.exact_count) atomicAdd(m.exact_count, (unsigned long long)exact);
 }
 

@@ -37,8 +37,8 @@ __global__ __launch_bounds__(256, dem_waves(kMode, 3)) void k_sim_terms(DemDev d
     const double Tx = tr.tx[i], Ty = tr.ty[i], Tz = tr.tz[i];
     const double Rx = tr.rx[i], Ry = tr.ry[i], Rz = tr.rz[i];
     double ax, ay, az, bx, by, bz;
-    fermat_solve(d, Tx, Ty, Tz, Px, Py, Pz, sa.n1, sa.n2, ax, ay, az);      // rdp:166-172
-    fermat_solve(d, Px, Py, Pz, Rx, Ry, Rz, sa.n2, sa.n1, bx, by, bz);      // rdp:175-181
+    fermat_solve<false, kMode == DEM_LDS>(d, Tx, Ty, Tz, Px, Py, Pz, sa.n1, sa.n2, ax, ay, az);      // rdp:166-172
+    fermat_solve<false, kMode == DEM_LDS>(d, Px, Py, Pz, Rx, Ry, Rz, sa.n2, sa.n1, bx, by, bz);      // rdp:175-181
     const double r1Tx = dist3(Tx, Ty, Tz, ax, ay, az);                      // rdp:197-222
     const double r2Tx = dist3(Px, Py, Pz, ax, ay, az);
     const double r1Rx = dist3(Rx, Ry, Rz, bx, by, bz);
