@@ -114,7 +114,9 @@ __device__ __forceinline__ double dem_interp(const DemDev& d, double x, double y
 // ulp), at most 4.8e-7 relative in all; kDefl deflates them by 8e-6, more
 // than an order of magnitude beyond that.  Estimates: with u = 2^-24, the FP32 inputs (Z, the
 // bilinear weights, z1, z2, the squared horizontal offsets, n1, n2: one
-// rounding each), the bilinear blend (~30 u |Z|max absolute), the sums and
+// rounding each; with the (height, x difference) pairs of an LDS-staged DEM
+// the difference is rounded once from FP64 instead of formed from two
+// rounded heights), the bilinear blend (~30 u |Z|max absolute), the sums and
 // v_sqrt_f32 (1 ulp) give
 //   |est - exact| <= 13 u est + 3 (n1 + n2) 31 u (|Z|max + |z1| + |z2|),
 // i.e. below kRho32 * est + D32 with kRho32 = 4e-6 (5x) and
