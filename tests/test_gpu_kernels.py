@@ -116,7 +116,7 @@ def test_upsample_batch_vs_oracle(ctx, oracle):
     assert bits_equal(ctx.upsample(root, 4).view(np.float64), oracle.upsample(root, 4).view(np.float64))
 
 
-@pytest.mark.parametrize("S", [64, 1024, 8192, 16384])
+@pytest.mark.parametrize("S", [2, 8, 64, 1024, 8192, 16384, 32768])   # LDS, split (16384) and global (32768) paths
 @pytest.mark.parametrize("conj", [True, False])
 def test_range_compress_fp64(ctx, oracle, S, conj):
     rng = np.random.default_rng(S + conj)
@@ -129,7 +129,7 @@ def test_range_compress_fp64(ctx, oracle, S, conj):
     assert np.abs(gpu - want).max() <= 1e-12 * np.abs(want).max()
 
 
-@pytest.mark.parametrize("S", [4096, 16384, 32768])
+@pytest.mark.parametrize("S", [16, 4096, 16384, 32768, 65536])   # LDS, split (32768) and global (65536) paths
 def test_range_compress_fp32(bsar, ctx, oracle, S):
     rng = np.random.default_rng(S)
     N = 4
