@@ -126,7 +126,16 @@ def test_range_compress_fp64(ctx, oracle, S, conj):
     gpu = ctx.range_compress(sig, ref, conj=conj)
     f = oracle.fft_correlation if conj else oracle.fft_convolution
     want = np.stack([f(s, ref) for s in sig])
-    assert np.abs(gpu - want).max() <= 1e-12 * np.abs(want).max()
+    # The reference's twiddles come from repeated multiplication w *= wlen
+    # (fft_utils.cpp:40), whose rounding grows linearly with the block length:
+    # the oracle, which keeps that recurrence, sits about 3e-17 * S of the peak
+    # away from an exactly-twiddled transform (1e-12 at S = 32768).  The GPU
+    # computes each twiddle directly, so it is held to max(1e-12, 1e-16 * S)
+    # of the oracle and to 1e-13 of numpy's transform.
+    assert np.abs(gpu - want).max() <= max(1e-12, 1e-16 * S) * np.abs(want).max()
+    H = np.fft.fft(ref)
+    exact = np.fft.ifft(np.fft.fft(sig, axis=1) * (np.conj(H) if conj else H), axis=1)
+    assert np.abs(gpu - exact).max() <= 1e-13 * np.abs(exact).max()
 
 
 @pytest.mark.parametrize("S", [16, 4096, 16384, 32768, 65536])   # LDS, split (32768) and global (65536) paths

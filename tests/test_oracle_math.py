@@ -24,6 +24,22 @@ def test_fft_matches_dft(oracle, n):
     assert np.isclose(np.sum(np.abs(X) ** 2), n * np.sum(np.abs(x) ** 2), rtol=1e-12)
 
 
+@pytest.mark.parametrize("S", [1024, 32768])
+def test_fft_recurrence_error_grows_with_length(oracle, S):
+    """The reference's twiddle recurrence w *= wlen (fft_utils.cpp:40) rounds
+    once per butterfly of a block, so the oracle's correlation drifts from an
+    exactly-twiddled one by O(eps * S): within 1e-16 * S of the peak, and at
+    S = 32768 above the 1e-13 an exact-twiddle transform keeps."""
+    rng = np.random.default_rng(S)
+    s = rng.normal(size=S) + 1j * rng.normal(size=S)
+    r = rng.normal(size=S) + 1j * rng.normal(size=S)
+    exact = np.fft.ifft(np.fft.fft(s) * np.conj(np.fft.fft(r)))
+    err = np.abs(oracle.fft_correlation(s, r) - exact).max() / np.abs(exact).max()
+    assert err <= 1e-16 * S
+    if S >= 32768:
+        assert err > 1e-13
+
+
 def test_fft_non_power_of_two_is_noop(oracle):
     x = np.arange(6) + 1j
     assert np.array_equal(oracle.fft(x), x)
