@@ -10,8 +10,51 @@
 
 namespace bsar {
 
+// Square root within 0.51 ulp of the exact root for a in [2^-1000, DBL_MAX]:
+// v_rsq_f64 (relative error e below 2^-22) refined by one Goldschmidt step
+// (error 1.5 e^2) and one Newton step on the square (error ~ e * 1.5 e^2,
+// plus the final rounding).  A zero or subnormal a gives NaN, +inf gives NaN,
+// NaN stays NaN.
+__device__ __forceinline__ double sqrt_faithful(double a)
+{
+    const double y = __builtin_amdgcn_rsq(a);
+    const double s0 = a * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, s0, 0.5);
+    const double s1 = __builtin_fma(s0, r, s0);
+    const double d = __builtin_fma(-s1, s1, a);
+    return __builtin_fma(d, h, s1);
+}
+
+// Mean distance from the n antennas (ax, ay, az) to q (proc:518-548): the
+// squared distances in the reference's operation order, summed in antenna
+// order, with faithful square roots.  When a squared distance is zero or
+// subnormal (sqrt_faithful's NaN) the sum is redone with the correctly
+// rounded root; a NaN or infinite distance makes the mean NaN or infinite
+// in both forms, and either way the pulse fails the range-window test.
+__device__ __forceinline__ double r1_mean_fp64(const double* __restrict__ ax, const double* __restrict__ ay,
+                                               const double* __restrict__ az, int n, double qx, double qy, double qz)
+{
+    double s = 0.0, amin = __builtin_inf();
+#pragma unroll 4
+    for (int j = 0; j < n; ++j) {
+        const double f0 = ax[j] - qx, f1 = ay[j] - qy, f2 = az[j] - qz;
+        const double a = f0 * f0 + f1 * f1 + f2 * f2;
+        amin = fmin(amin, a);
+        s += sqrt_faithful(a);
+    }
+    if (!(amin >= 0x1p-1000)) {
+        s = 0.0;
+        for (int j = 0; j < n; ++j) {
+            const double f0 = ax[j] - qx, f1 = ay[j] - qy, f2 = az[j] - qz;
+            s += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+        }
+    }
+    return s / (double)n;
+}
+
 // Plain FP64 body, proc_test_script.cpp:484-668 in the reference's operation
-// order (built with -ffp-contract=off).
+// order (built with -ffp-contract=off); the R1 means take faithful square
+// roots (r1_mean_fp64).
 template <bool kSplit = false, bool kPairs = false>
 __device__ __forceinline__ void bp_fp64_thread(const DemDev& d, const BpArgs& a, int g, int64_t v, int p0, int p1)
 {
@@ -21,7 +64,6 @@ __device__ __forceinline__ void bp_fp64_thread(const DemDev& d, const BpArgs& a,
     const double Px = a.gx[xi], Py = a.gy[yi], Pz = a.gz[zi];
     const double n1 = a.n1, n2 = a.n2;
     const int n = a.traj.n;
-    const double dn = (double)n;
     double2* accp = a.acc + (int64_t)g * a.acc_stride + v;
     double accr = accp->x, acci = accp->y;
 
@@ -33,16 +75,8 @@ __device__ __forceinline__ void bp_fp64_thread(const DemDev& d, const BpArgs& a,
         const double R2t = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:496-505
         e0 = Px - qbx; e1 = Py - qby; e2 = Pz - qbz;
         const double R2r = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:507-516
-        double st = 0.0, sr = 0.0;
-        for (int j = 0; j < n; ++j) {                                                                     // proc:518-532
-            const double f0 = a.traj.tx[j] - qax, f1 = a.traj.ty[j] - qay, f2 = a.traj.tz[j] - qaz;
-            st += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
-        }
-        for (int j = 0; j < n; ++j) {                                                                     // proc:534-548
-            const double f0 = a.traj.rx[j] - qbx, f1 = a.traj.ry[j] - qby, f2 = a.traj.rz[j] - qbz;
-            sr += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
-        }
-        const double R1t = st / dn, R1r = sr / dn;
+        const double R1t = r1_mean_fp64(a.traj.tx, a.traj.ty, a.traj.tz, n, qax, qay, qaz);              // proc:518-532
+        const double R1r = r1_mean_fp64(a.traj.rx, a.traj.ry, a.traj.rz, n, qbx, qby, qbz);              // proc:534-548
         const double path = n1 * R1t + n2 * R2t + n2 * R2r + n1 * R1r;
         const double tv = a.inv_c * path;                                                                // proc:567
         const double rngBin = 1.0 + (tv * a.fs4);                                                        // proc:571
