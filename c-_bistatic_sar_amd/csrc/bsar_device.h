@@ -43,6 +43,7 @@ __device__ __forceinline__ void axis_lookup(const double* A, int n, double a0, d
                                             double x, int& i0, int& di, double& w)
 {
     int found = -1;
+    double lo = 0.0, hi = 0.0;   // A[found], A[found + 1]
     if (sorted) {
         // A strictly increasing: with l = #{A[k] < x}, the only candidate is
         // max(l-1, 0); test it with the reference predicate itself.
@@ -51,21 +52,40 @@ __device__ __forceinline__ void axis_lookup(const double* A, int n, double a0, d
         if (!(gd > 0.0)) g = 0;
         else if (gd >= (double)n) g = n;
         else g = (int)gd;
-        while (g > 0 && A[g - 1] >= x) --g;
-        while (g < n && A[g] < x) ++g;
-        int c = g > 0 ? g - 1 : 0;
-        if (c <= n - 2 && A[c] <= x && x <= A[c + 1]) found = c;
+        // Inside the axis the uniform-spacing guess is nearly always l: both
+        // nodes around it load in one round trip, and A[g-1] < x <= A[g]
+        // confirms l = g and the interval g - 1 at once.
+        const int gl = g > 0 ? g - 1 : 0, gh = g < n ? g : n - 1;
+        lo = A[gl];
+        hi = A[gh];
+        if (g > 0 && g < n && lo < x && x <= hi) {
+            found = g - 1;
+        } else {
+            while (g > 0 && A[g - 1] >= x) --g;
+            while (g < n && A[g] < x) ++g;
+            int c = g > 0 ? g - 1 : 0;
+            if (c <= n - 2 && A[c] <= x && x <= A[c + 1]) {
+                found = c;
+                lo = A[c];
+                hi = A[c + 1];
+            }
+        }
     } else {
         for (int i = 0; i + 1 < n; ++i) {
-            if (x >= A[i] && x <= A[i + 1]) { found = i; break; }
+            if (x >= A[i] && x <= A[i + 1]) {
+                found = i;
+                lo = A[i];
+                hi = A[i + 1];
+                break;
+            }
         }
     }
     double ww;
     if (found >= 0) {
         i0 = found; di = 1;
-        double d = A[found + 1] - A[found];
+        double d = hi - lo;
         if (d == 0) d = 1.0;
-        ww = (x - A[found]) / d;
+        ww = (x - lo) / d;
     } else {
         i0 = 0; di = 0;
         ww = x - A[0];           // (x - A[0]) / 1.0
@@ -181,6 +201,9 @@ __device__ __forceinline__ int node_floor(const double* A, int n, double a0, dou
 {
     const double gd = (x - a0) * ih;
     int i = !(gd > 0.0) ? 0 : (gd >= (double)(n - 1) ? n - 1 : (int)gd);
+    // the guess and its right neighbour in one round trip; walk only if wrong
+    const double a = A[i], b = A[i + 1 < n ? i + 1 : i];
+    if (a <= x && (i + 1 >= n || b > x)) return i;
     while (i > 0 && A[i] > x) --i;
     while (i + 1 < n && A[i + 1] <= x) ++i;
     return i;
@@ -189,6 +212,9 @@ __device__ __forceinline__ int node_ceil(const double* A, int n, double a0, doub
 {
     const double gd = (x - a0) * ih;
     int i = !(gd > 0.0) ? 0 : (gd >= (double)(n - 1) ? n - 1 : (int)gd);
+    // the guess and its left neighbour in one round trip; walk only if wrong
+    const double a = A[i], b = A[i > 0 ? i - 1 : 0];
+    if (a >= x && (i == 0 || b < x)) return i;
     while (i + 1 < n && A[i] < x) ++i;
     while (i > 0 && A[i - 1] >= x) --i;
     return i;
