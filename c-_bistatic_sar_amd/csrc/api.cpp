@@ -107,6 +107,8 @@ int upload_dem(bsar_ctx* ctx, const bsar_dem* dem, double** dX, double** dY, dou
     if (!empty && strictly_increasing(Y.data(), dem->ny)) d.flags |= DEM_Y_SORTED;
     d.x0 = X[0];
     d.y0 = Y[0];
+    d.xn = X[nx - 1];
+    d.yn = Y[ny - 1];
     d.ihx = (dem->nx > 1 && (d.flags & DEM_X_SORTED)) ? (dem->nx - 1) / (X[dem->nx - 1] - X[0]) : 0.0;
     d.ihy = (dem->ny > 1 && (d.flags & DEM_Y_SORTED)) ? (dem->ny - 1) / (Y[dem->ny - 1] - Y[0]) : 0.0;
     d.zlo = lo;

@@ -28,6 +28,7 @@ struct DemDev {
     const float2* Zp;          // (Z, Z[.][i+1] - Z[.][i]) in FP32 per node, same layout (screening, DEM in LDS)
     int nx, ny, zs, flags;
     double x0, ihx, y0, ihy;   // first value and inverse mean spacing (index guess only)
+    double xn, yn;             // last values X[nx-1], Y[ny-1]
     double zlo, zhi;           // height range of the stored Z (0, 0 when empty)
     double zabs;               // max |Z|
 };
@@ -195,29 +196,46 @@ __device__ __forceinline__ Leg make_leg(const DemDev& d, double x1, double y1, d
 __device__ __forceinline__ int center_out(int k, int c) { return (k & 1) ? c - ((k + 1) >> 1) : c + (k >> 1); }
 
 // Largest i with A[i] <= x (0 when none) and smallest i with A[i] >= x
-// (n - 1 when none) on a strictly increasing axis, from the uniform-spacing
-// guess.
-__device__ __forceinline__ int node_floor(const double* A, int n, double a0, double ih, double x)
+// (n - 1 when none) on a strictly increasing axis, walked from any start i.
+__device__ __forceinline__ int floor_walk(const double* A, int n, double x, int i)
 {
-    const double gd = (x - a0) * ih;
-    int i = !(gd > 0.0) ? 0 : (gd >= (double)(n - 1) ? n - 1 : (int)gd);
-    // the guess and its right neighbour in one round trip; walk only if wrong
-    const double a = A[i], b = A[i + 1 < n ? i + 1 : i];
-    if (a <= x && (i + 1 >= n || b > x)) return i;
     while (i > 0 && A[i] > x) --i;
     while (i + 1 < n && A[i + 1] <= x) ++i;
     return i;
 }
-__device__ __forceinline__ int node_ceil(const double* A, int n, double a0, double ih, double x)
+__device__ __forceinline__ int ceil_walk(const double* A, int n, double x, int i)
 {
-    const double gd = (x - a0) * ih;
-    int i = !(gd > 0.0) ? 0 : (gd >= (double)(n - 1) ? n - 1 : (int)gd);
-    // the guess and its left neighbour in one round trip; walk only if wrong
-    const double a = A[i], b = A[i > 0 ? i - 1 : 0];
-    if (a >= x && (i == 0 || b < x)) return i;
     while (i + 1 < n && A[i] < x) ++i;
     while (i > 0 && A[i - 1] >= x) --i;
     return i;
+}
+
+// floor_walk of xa, ya and ceil_walk of xb, yb from the uniform-spacing
+// guesses (floor and ceiling of the fractional index).  The eight nodes that
+// confirm the four guesses load together, so when every guess is right the
+// window costs one LDS round trip; a wrong guess walks.
+__device__ __forceinline__ void window_nodes(const DemDev& d, double xa, double xb, double ya, double yb,
+                                             int& i0, int& i1, int& j0, int& j1)
+{
+    auto gfloor = [](int n, double a0, double ih, double x) {
+        const double gd = (x - a0) * ih;
+        return !(gd > 0.0) ? 0 : (gd >= (double)(n - 1) ? n - 1 : (int)gd);
+    };
+    auto gceil = [](int n, double a0, double ih, double x) {
+        const double gd = (x - a0) * ih;
+        return !(gd > 0.0) ? 0 : (gd >= (double)(n - 1) ? n - 1 : min((int)gd + 1, n - 1));
+    };
+    int a = gfloor(d.nx, d.x0, d.ihx, xa), b = gceil(d.nx, d.x0, d.ihx, xb);
+    int c = gfloor(d.ny, d.y0, d.ihy, ya), e = gceil(d.ny, d.y0, d.ihy, yb);
+    const double pa = d.X[a], qa = d.X[min(a + 1, d.nx - 1)];
+    const double pb = d.X[b], qb = d.X[max(b - 1, 0)];
+    const double pc = d.Y[c], qc = d.Y[min(c + 1, d.ny - 1)];
+    const double pe = d.Y[e], qe = d.Y[max(e - 1, 0)];
+    if (!(pa <= xa && (a + 1 >= d.nx || qa > xa))) a = floor_walk(d.X, d.nx, xa, a);
+    if (!(pb >= xb && (b == 0 || qb < xb))) b = ceil_walk(d.X, d.nx, xb, b);
+    if (!(pc <= ya && (c + 1 >= d.ny || qc > ya))) c = floor_walk(d.Y, d.ny, ya, c);
+    if (!(pe >= yb && (e == 0 || qe < yb))) e = ceil_walk(d.Y, d.ny, yb, e);
+    i0 = a; i1 = b; j0 = c; j1 = e;
 }
 
 // Squared vertical gaps from z1 / z2 to the heights the DEM lookup can return
@@ -228,16 +246,33 @@ __device__ __forceinline__ int node_ceil(const double* A, int n, double a0, doub
 __device__ __forceinline__ void local_gaps(const DemDev& d, double z1, double z2, double xa, double xb, double ya,
                                            double yb, double& g1sq, double& g2sq)
 {
-    const int i0 = node_floor(d.X, d.nx, d.x0, d.ihx, xa), i1 = node_ceil(d.X, d.nx, d.x0, d.ihx, xb);
-    const int j0 = node_floor(d.Y, d.ny, d.y0, d.ihy, ya), j1 = node_ceil(d.Y, d.ny, d.y0, d.ihy, yb);
+    int i0, i1, j0, j1;
+    window_nodes(d, xa, xb, ya, yb, i0, i1, j0, j1);
     if (i1 < i0 || j1 < j0 || (i1 - i0 + 1) * (j1 - j0 + 1) > 16) return;
-    const bool wx0 = xa < d.X[0] || xb > d.X[d.nx - 1];
-    const bool wy0 = ya < d.Y[0] || yb > d.Y[d.ny - 1];
+    const bool wx0 = xa < d.x0 || xb > d.xn;
+    const bool wy0 = ya < d.y0 || yb > d.yn;
     double lo = __builtin_inf(), hi = -__builtin_inf();
-    for (int j = j0; j <= j1; ++j) {
-        const double* r = d.Z + j * d.zs;
-        for (int i = i0; i <= i1; ++i) { lo = fmin(lo, r[i]); hi = fmax(hi, r[i]); }
-        if (wx0) { lo = fmin(lo, r[0]); hi = fmax(hi, r[0]); }
+    if (i1 - i0 < 4 && j1 - j0 < 4) {
+        // up to 4 x 4 nodes in one round trip: the clamped indices repeat a
+        // node of the window, which leaves the minimum and maximum unchanged
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const double* r = d.Z + min(j0 + b, j1) * d.zs;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const double v = r[min(i0 + a, i1)];
+                lo = fmin(lo, v);
+                hi = fmax(hi, v);
+            }
+        }
+    } else {
+        for (int j = j0; j <= j1; ++j) {
+            const double* r = d.Z + j * d.zs;
+            for (int i = i0; i <= i1; ++i) { lo = fmin(lo, r[i]); hi = fmax(hi, r[i]); }
+        }
+    }
+    if (wx0) {
+        for (int j = j0; j <= j1; ++j) { lo = fmin(lo, d.Z[j * d.zs]); hi = fmax(hi, d.Z[j * d.zs]); }
     }
     if (wy0) {
         for (int i = i0; i <= i1; ++i) { lo = fmin(lo, d.Z[i]); hi = fmax(hi, d.Z[i]); }
