@@ -167,6 +167,21 @@ __device__ __forceinline__ double lower_bound32(double a, double b, float g1f, f
     return (double)(lb * (float)kDefl);
 }
 
+// Slope of the one-axis bound l(q) = n1 sqrt((q - c1)^2 + g1sq) + n2 sqrt((q - c2)^2 + g2sq)
+// from the signed offsets u1 = q - c1, u2 = q - c2, in FP32.  Each term
+// n u / sqrt(u^2 + g) lies in [-n, n]; the roundings of u, g, n, the fma,
+// v_rsq_f32 and the products keep its error below 8 * 2^-24 * n, so the sum
+// is within 5e-7 (n1 + n2) of the exact slope -- 20x inside kSlope.  u = g = 0
+// gives 0 * inf = NaN, which certifies nothing.
+constexpr float kSlope = 1e-5f;
+
+__device__ __forceinline__ float slope32(double u1, double u2, float g1f, float g2f, float n1f, float n2f)
+{
+    const float a = (float)u1, b = (float)u2;
+    const float s1 = __builtin_fmaf(a, a, g1f), s2 = __builtin_fmaf(b, b, g2f);
+    return __builtin_fmaf(n2f * b, __builtin_amdgcn_rsqf(s2), (n1f * a) * __builtin_amdgcn_rsqf(s1));
+}
+
 struct Leg {
     double x1, y1, z1, x2, y2, z2, n1, n2;
     double g1sq, g2sq;   // squared vertical gaps from z1 / z2 to the DEM height range
@@ -317,7 +332,7 @@ __device__ __forceinline__ double exact_opt(const DemDev& d, const Leg& L, bool 
     return L.n1 * d1 + L.n2 * d2;
 }
 
-template <int G, bool kSplit, bool kPairs>
+template <int G, bool kSplit, bool kPairs, bool kSide>
 __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, double R, double& bestX, double& bestY,
                                              double& minOpt)
 {
@@ -362,6 +377,20 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     // element offsets of the rows j0 and j0 + dj, weight, squared y offsets.
     // A dropped column keeps sy1 = +inf, which makes its estimate +inf (or
     // NaN), so the screening loop needs no per-candidate mask test.
+    //
+    // Side stop.  The one-axis bound is convex in the column (row) position,
+    // so once it prunes a column whose bound still rises in the direction of
+    // travel -- its slope, taken epsy before the column's true position,
+    // exceeds kSlope (n1 + n2) there -- every column further out on that side
+    // has an exact bound at least as large, above M now and later: those
+    // columns are dropped without evaluating their bounds (bit 0 of cdone the
+    // + side, even k; bit 1 the - side, odd k).  Rows do the same.  What is
+    // dropped this way could not win or tie, so the argmin is unchanged.
+    // kSide: on in the back-projection kernels, where the saved bound
+    // evaluations pay; the search-only kernels, at 3 waves per SIMD, would
+    // spill the extra state.
+    const float slope = kSlope * (L.n1f + L.n2f);
+    unsigned cdone = 0u;
     auto part = [&](auto jb, auto nj) {
     constexpr int JB = decltype(jb)::value, NJ = decltype(nj)::value;
     int o0[NJ], o1[NJ];
@@ -369,15 +398,25 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     unsigned colmask = 0u;
 #pragma unroll
     for (int t = 0; t < NJ; ++t) {
-        const int j = center_out(JB + t, C);
+        const int kk = JB + t;
+        const int j = center_out(kk, C);
         o0[t] = 0;
         o1[t] = 0;
         wyf[t] = 0.0f;
         sy1[t] = __builtin_inff();
         sy2[t] = 0.0f;
+        const unsigned side = (kk & 1) ? 2u : 1u;
+        if (kSide && kk > 0 && (cdone & side)) continue;
         const double pya = __builtin_fma((double)j, stepY, minY);
         const double a = fmax(fabs(pya - L.y1) - epsy, 0.0), b = fmax(fabs(pya - L.y2) - epsy, 0.0);
-        if (lower_bound32(a, b, g1f, g2f, L.n1f, L.n2f) > M) continue;
+        if (lower_bound32(a, b, g1f, g2f, L.n1f, L.n2f) > M) {
+            if (kSide && kk > 0) {
+                const double q = (kk & 1) ? pya + epsy : pya - epsy;
+                const float sl = slope32(q - L.y1, q - L.y2, g1f, g2f, L.n1f, L.n2f);
+                if ((kk & 1) ? sl < -slope : sl > slope) cdone |= side;
+            }
+            continue;
+        }
         colmask |= 1u << t;
         const double py = minY + j * spanY / (G - 1);
         int j0 = 0, dj = 0;
@@ -395,15 +434,28 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     const float k3 = (float)(1.0 + 3.0 * kRho32), k2 = (float)(1.0 + 2.0 * kRho32);
     const float Dk3 = (float)(L.D * (1.0 + 3.0 * kRho32)), D2 = (float)(2.0 * L.D);
 
+    unsigned rdone = 0u;   // side stop of the rows (bit 0: + side, bit 1: - side)
 #pragma unroll 1
     for (int k = 0; k < G; ++k) {
         const int i = center_out(k, C);
         // row lower bound from an approximate px (|error| < epsx)
         {
+            const unsigned side = (k & 1) ? 2u : 1u;
+            if (kSide && k > 0 && (rdone & side)) {
+                if (rdone == 3u) break;
+                continue;
+            }
             const double pxa = __builtin_fma((double)i, stepX, minX);
             const double a1 = fmax(fabs(pxa - L.x1) - epsx, 0.0);
             const double a2 = fmax(fabs(pxa - L.x2) - epsx, 0.0);
-            if (lower_bound32(a1, a2, g1f, g2f, L.n1f, L.n2f) > M) continue;
+            if (lower_bound32(a1, a2, g1f, g2f, L.n1f, L.n2f) > M) {
+                if (kSide && k > 0) {
+                    const double q = (k & 1) ? pxa + epsx : pxa - epsx;
+                    const float sl = slope32(q - L.x1, q - L.x2, g1f, g2f, L.n1f, L.n2f);
+                    if ((k & 1) ? sl < -slope : sl > slope) rdone |= side;
+                }
+                continue;
+            }
         }
         // the probe (i, j) = (C, C) is bit 0 of row C in the first part
         const unsigned colok = (JB == 0 && i == C) ? (colmask & ~1u) : colmask;
@@ -490,7 +542,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
 }
 
 // calculateRefractionPoint, raw_data_processing.cpp:455-572.
-template <bool kSplit = false, bool kPairs = false>
+template <bool kSplit = false, bool kPairs = false, bool kSide = kSplit>
 __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double y1, double z1,
                                              double x2, double y2, double z2, double n1, double n2,
                                              double& qx, double& qy, double& qz)
@@ -501,11 +553,11 @@ __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double 
     double R = std_max(__builtin_sqrt(ex * ex + ey * ey), 500.0);    // :483-484
     double minOpt = 1e9;                                             // :497
     const Leg L = make_leg(d, x1, y1, z1, x2, y2, z2, n1, n2);
-    fermat_level<15, kSplit, kPairs>(d, L, R, bestX, bestY, minOpt);
+    fermat_level<15, kSplit, kPairs, kSide>(d, L, R, bestX, bestY, minOpt);
     R /= 5.0;                                                        // :551
 #pragma unroll 1
     for (int level = 1; level < 5; ++level) {
-        fermat_level<11, kSplit, kPairs>(d, L, R, bestX, bestY, minOpt);
+        fermat_level<11, kSplit, kPairs, kSide>(d, L, R, bestX, bestY, minOpt);
         R /= 3.0;
     }
     qx = bestX;                                                      // :555-557
