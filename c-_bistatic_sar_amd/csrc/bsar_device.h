@@ -386,9 +386,9 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     // columns are dropped without evaluating their bounds (bit 0 of cdone the
     // + side, even k; bit 1 the - side, odd k).  Rows do the same.  What is
     // dropped this way could not win or tie, so the argmin is unchanged.
-    // kSide: on in the back-projection kernels, where the saved bound
-    // evaluations pay; the search-only kernels, at 3 waves per SIMD, would
-    // spill the extra state.
+    // kSide: on in the back-projection kernels with the DEM in LDS, where the
+    // saved bound evaluations pay; the search-only kernels, at 3 waves per
+    // SIMD, would spill the extra state.
     const float slope = kSlope * (L.n1f + L.n2f);
     unsigned cdone = 0u;
     auto part = [&](auto jb, auto nj) {
@@ -542,7 +542,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
 }
 
 // calculateRefractionPoint, raw_data_processing.cpp:455-572.
-template <bool kSplit = false, bool kPairs = false, bool kSide = kSplit>
+template <bool kSplit = false, bool kPairs = false, bool kSide = false>
 __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double y1, double z1,
                                              double x2, double y2, double z2, double n1, double n2,
                                              double& qx, double& qy, double& qz)
