@@ -8,7 +8,8 @@
 namespace bsar {
 
 // DEMs up to this many 8-byte slots (X + Y + padded Z in FP64, then the FP32
-// copy of Z) are staged in LDS per workgroup (DEM_LDS); up to kSplitMax
+// (height, x difference) pairs of Z) are staged in LDS per workgroup
+// (DEM_LDS); up to kSplitMax
 // slots of X, Y and the FP32 heights alone are staged with two workgroups per
 // CU, the FP64 heights read from global memory (DEM_SPLIT); larger DEMs are
 // read through the L1/L2 caches (DEM_GLOBAL).

@@ -90,6 +90,33 @@ def test_dem_variants(bsar, ctx, oracle, scene_mod, demkind):
     assert peak_err(img, want) <= FP64_TOL
 
 
+@pytest.mark.parametrize("demkind", ["fbm_small", "ragged", "dup", "flat", "small_z"])
+def test_dem_variants_staged_in_lds(bsar, ctx, oracle, scene_mod, golden, demkind):
+    """DEMs small enough for the back-projection kernels' LDS-staged search
+    (the 15-wide level in two column parts, (height, x difference) pairs,
+    the side stop of the row and column bounds): 40 m RMS terrain, a ragged
+    grid with non-uniform axes, duplicated nodes, a flat DEM, and Z smaller
+    than its axes.  Any refraction point that differs from the reference's
+    moves a path by centimetres, which the 1e-9 bar would show."""
+    if demkind == "fbm_small":
+        dem = scene_mod.dem_fbm(np.random.default_rng(5), nx=30, ny=28)
+    else:
+        dem = tuple(np.ascontiguousarray(golden[f"dem_{demkind}_{a}"]) for a in "xyz")
+    nx, ny = len(dem[0]), len(dem[1])
+    assert nx + ny + 2 * ny * (nx + 1) <= 8192    # 8-byte slots of the LDS-staged DEM
+    z0 = oracle.dem_interp(dem, [[20.0, 0.0]])[0]
+    t = Tiny(n=12, grid=(4, 3, 2), dem=dem, z0=z0)
+    root = t.random_root()
+    ds = ctx.dataset(t.params(bsar), bsar.Dem(*dem), t.tx, t.rx)
+    ds.load_rootdata(root)
+    want = oracle.backproject(t.bp(), dem, t.tx, t.rx, oracle.upsample(root), t.gx, t.gy, t.gz)
+    assert np.abs(want).max() > 0
+    img, _ = ds.backproject(t.gx, t.gy, t.gz, bsar.FP64)
+    assert peak_err(img, want) <= FP64_TOL
+    mixed, _ = ds.backproject(t.gx, t.gy, t.gz, bsar.MIXED)
+    assert db(peak_err(mixed, want)) <= MIXED_DB
+
+
 def test_degenerate_grids_and_state_errors(bsar, ctx, oracle):
     t = Tiny(n=8)
     ds = ctx.dataset(t.params(bsar), bsar.Dem(*t.dem), t.tx, t.rx)
