@@ -11,20 +11,15 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
-#include <fstream>
-#include <sstream>
 #include <string>
 #include <vector>
 
 #include "bsar.h"
 #include "bsar_facade.hpp"
+#include "proc_inputs.hpp"
 
 namespace fa = bsar::facade;
-
-struct Dem {
-    std::vector<double> X_vec, Y_vec;
-    std::vector<std::vector<double>> Z_DEM;
-};
+using namespace bsar_tools;
 
 static void usage()
 {
@@ -33,26 +28,6 @@ static void usage()
         "                 [--device N] [--check]\n"
         "  DIR defaults to build/output/proc (relative to the working directory, as the reference).\n"
         "  --check parses the inputs and prints their shapes without using the GPU.\n");
-}
-
-// initializeEnvironment, raw_data_processing.cpp:14-46: n1=..., n2=... lines,
-// defaults 1.0 / 4.0.
-static void read_environment(const std::string& path, double& n1, double& n2)
-{
-    n1 = 1.0;
-    n2 = 4.0;
-    std::ifstream f(path);
-    if (!f.is_open()) return;
-    std::string line;
-    while (std::getline(f, line)) {
-        if (line.empty()) continue;
-        std::istringstream iss(line);
-        std::string key, value;
-        if (std::getline(iss, key, '=') && std::getline(iss, value)) {
-            if (key == "n1") n1 = std::stod(value);
-            else if (key == "n2") n2 = std::stod(value);
-        }
-    }
 }
 
 int main(int argc, char** argv)
@@ -82,55 +57,26 @@ int main(int argc, char** argv)
         using clk = std::chrono::steady_clock;
         const auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
         const auto t_read = clk::now();
-        double n1, n2;
-        read_environment(dir + "/environment.txt", n1, n2);
-        fa::MatrixInfo info;
-        std::vector<std::complex<double>> root = fa::readBinary<double>(dir + "/rootData_complex.bin", info);
-        if (info.dimensions.size() != 2) {
-            std::fprintf(stderr, "[ERROR] rootData must be a 2-D matrix\n");
+        ProcInputs in;
+        const std::string bad = read_proc_inputs(dir, in);
+        if (!bad.empty()) {
+            std::fprintf(stderr, "[ERROR] %s\n", bad.c_str());
             return 1;
         }
-        const size_t S = info.dimensions[0], Nd = info.dimensions[1];
-        std::vector<double> gx = fa::readCSV(dir + "/grid_xAxis.csv");
-        std::vector<double> gy = fa::readCSV(dir + "/grid_yAxis.csv");
-        std::vector<double> gz = fa::readCSV(dir + "/grid_zAxis.csv");
-        Dem dem;
-        dem.X_vec = fa::readCSV(dir + "/DEM_X_vec.csv");
-        dem.Y_vec = fa::readCSV(dir + "/DEM_Y_vec.csv");
-        dem.Z_DEM = fa::readCSVMatrix(dir + "/DEM_Z_matrix.csv");
-        std::vector<std::vector<double>> Tx = fa::readCSVMatrix(dir + "/Tx_positions.csv");
-        std::vector<std::vector<double>> Rx = fa::readCSVMatrix(dir + "/Rx_positions.csv");
-        for (const auto* t : {&Tx, &Rx})
-            for (const auto& r : *t)
-                if (r.size() < 3) { std::fprintf(stderr, "[ERROR] antenna rows need 3 values\n"); return 1; }
-        if (Tx.size() != Rx.size()) { std::fprintf(stderr, "[ERROR] Tx and Rx counts differ\n"); return 1; }
-        std::printf("inputs: rootData %zu x %zu, grid %zu x %zu x %zu, DEM %zu x %zu, %zu positions, n1=%g n2=%g\n",
-                    S, Nd, gx.size(), gy.size(), gz.size(), dem.X_vec.size(), dem.Y_vec.size(), Tx.size(), n1, n2);
+        print_proc_inputs(in);
         if (check_only) return 0;
-        if (S < 2 || Nd < 1) { std::fprintf(stderr, "[ERROR] rootData needs S >= 2 and >= 1 pulse\n"); return 1; }
+        if (in.S < 2 || in.Nd < 1) { std::fprintf(stderr, "[ERROR] rootData needs S >= 2 and >= 1 pulse\n"); return 1; }
         const double sec_read = since(t_read);
         const auto t_load = clk::now();
-
-        const size_t n = Tx.size();
-        std::vector<double> tx(3 * n), rx(3 * n);
-        for (size_t i = 0; i < n; ++i)
-            for (int k = 0; k < 3; ++k) { tx[3 * i + k] = Tx[i][k]; rx[3 * i + k] = Rx[i][k]; }
-        fa::FlatDem fd(dem);
         bsar_ctx* ctx = fa::context();
-        bsar_params p{fc, c, fs, n1, n2, factor, 0};
-        bsar_dataset* ds = nullptr;
-        fa::check(bsar_dataset_create(ctx, &p, &fd.d, tx.data(), rx.data(), (int)n, &ds), ctx, "dataset");
-        // rootData flat index r + p*S is already pulse-major, range fastest
-        int st = bsar_dataset_load_rootdata(ctx, ds, reinterpret_cast<const double*>(root.data()), (int)S, (int)Nd);
-        if (st != BSAR_OK) { bsar_dataset_destroy(ds); fa::check(st, ctx, "rootData upload"); }
-        fa::check(bsar_ctx_synchronize(ctx), ctx, "rootData upload");
+        bsar_dataset* ds = upload_proc_dataset(ctx, in, fc, fs, c, factor);
         const double sec_load = since(t_load);
-        const size_t nx = gx.size(), ny = gy.size(), nz = gz.size();
+        const size_t nx = in.gx.size(), ny = in.gy.size(), nz = in.gz.size();
         std::vector<std::complex<double>> img(nx * ny * nz);
         bsar_report rep{};
         const auto t0 = clk::now();
-        st = bsar_backproject(ctx, ds, gx.data(), (int)nx, gy.data(), (int)ny, gz.data(), (int)nz, precision,
-                              reinterpret_cast<double*>(img.data()), &rep);
+        const int st = bsar_backproject(ctx, ds, in.gx.data(), (int)nx, in.gy.data(), (int)ny, in.gz.data(), (int)nz,
+                                        precision, reinterpret_cast<double*>(img.data()), &rep);
         const double wall = since(t0);
         bsar_dataset_destroy(ds);
         fa::check(st, ctx, "backproject");
@@ -139,13 +85,13 @@ int main(int argc, char** argv)
                     precision == BSAR_MIXED ? "mixed" : "fp64");
         const auto t_write = clk::now();
         const std::string out = dir + "/output_3D_matrix.bin";
-        st = bsar_write_image(out.c_str(), reinterpret_cast<const double*>(img.data()), (int)nx, (int)ny, (int)nz,
-                              gx.data(), gy.data(), gz.data());
-        fa::check(st, nullptr, "export");
+        fa::check(bsar_write_image(out.c_str(), reinterpret_cast<const double*>(img.data()), (int)nx, (int)ny, (int)nz,
+                                   in.gx.data(), in.gy.data(), in.gz.data()),
+                  nullptr, "export");
         std::printf("wrote %s\n", out.c_str());
         std::printf("[PERFORMANCE] stages: read inputs %.3f s (rootData %.1f MB), upload + x%d spline %.3f s, "
                     "back-projection %.3f s (image to host included), write image %.3f s (%.1f MB)\n",
-                    sec_read, S * Nd * 16 / 1e6, factor, sec_load, wall, since(t_write), nx * ny * nz * 16 / 1e6);
+                    sec_read, in.S * in.Nd * 16 / 1e6, factor, sec_load, wall, since(t_write), nx * ny * nz * 16 / 1e6);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[ERROR] %s\n", e.what());
         return 1;

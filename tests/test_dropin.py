@@ -136,3 +136,81 @@ def test_sim_then_proc_round_trip(tmp_path, bsar, oracle):
     want = oracle.backproject(bp, dem, tx, rx, oracle.upsample(root.reshape(N, S)), gx, gy, gz)
     assert np.abs(img - want).max() <= 1e-9 * np.abs(want).max()
     assert np.abs(want).max() > 0
+
+
+SERVE = PKG_DIR / "bsar_serve"
+SERVE_JOBS = ("# tag ix0 ix1 iy0 iy1 iz0 iz1 pulses precision\n"
+              "full 0 5 0 4 0 3 0 fp64\n"
+              "sub_mixed 1 4 0 3 1 3 10 mixed\n"
+              "\n"
+              "sub_fp64 0 2 1 4 0 2 12 fp64\n")
+
+
+def run_serve(*args, stdin=None, cwd=None):
+    return subprocess.run([str(SERVE), *map(str, args)], input=stdin, capture_output=True, text=True, timeout=300,
+                          cwd=cwd)
+
+
+def test_serve_parses_inputs_and_jobs(tmp_path):
+    g = np.load(GOLDEN / "ref_bp_random.npz")
+    write_proc_dir(tmp_path, g)
+    (tmp_path / "jobs.txt").write_text(SERVE_JOBS)
+    r = run_serve("--check", "--dir", tmp_path, "--jobs", tmp_path / "jobs.txt")
+    assert r.returncode == 0, r.stderr
+    assert "jobs: 3" in r.stdout
+    assert "job full: 5 x 4 x 3 voxels x 16 pulses, fp64" in r.stdout
+    assert "job sub_mixed: 3 x 3 x 2 voxels x 10 pulses, mixed" in r.stdout
+    r = run_serve("--check", "--dir", tmp_path, stdin=SERVE_JOBS)      # job list on standard input
+    assert r.returncode == 0 and "jobs: 3" in r.stdout
+
+
+@pytest.mark.parametrize("line", ["a 0 6 0 4 0 3 0 fp64", "a 2 2 0 4 0 3 0 fp64", "a 0 5 0 4 0 3 17 fp64",
+                                  "a 0 5 0 4 0 3 0 fp32", "a 0 5 0 4 0 3 fp64", "a/b 0 5 0 4 0 3 0 fp64",
+                                  "a 0 5 0 4 0 3 0 fp64\na 0 1 0 1 0 1 0 mixed", "# nothing", "a 0 5 0 4 0 3 0 fp64 x"])
+def test_serve_rejects_malformed_jobs(tmp_path, line):
+    g = np.load(GOLDEN / "ref_bp_random.npz")
+    write_proc_dir(tmp_path, g)
+    r = run_serve("--check", "--dir", tmp_path, stdin=line + "\n")
+    assert r.returncode == 2 and "[ERROR]" in r.stderr
+
+
+def test_serve_missing_input_exits_1(tmp_path):
+    g = np.load(GOLDEN / "ref_bp_random.npz")
+    write_proc_dir(tmp_path, g)
+    (tmp_path / "DEM_Z_matrix.csv").unlink()
+    r = run_serve("--check", "--dir", tmp_path, stdin=SERVE_JOBS)
+    assert r.returncode == 1 and "[ERROR]" in r.stderr
+
+
+@pytest.mark.gpu
+def test_serve_images_match_backproject(tmp_path, bsar, ctx):
+    """bsar_serve images each job of the list through one server; every image
+    equals bsar_backproject over the same sub-grid with a dataset of the job's
+    first K positions, bit for bit, and the full-grid job matches the
+    reference's own output_3D_matrix.bin."""
+    g = np.load(GOLDEN / "ref_bp_random.npz")
+    write_proc_dir(tmp_path, g)
+    r = run_serve("--dir", tmp_path, "--round-ms", "5", stdin=SERVE_JOBS)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "[PERFORMANCE] serving: 3 jobs" in r.stdout
+    rd = lambda n: np.array([[float(x) for x in ln.split(",")] for ln in (tmp_path / n).read_text().split()])
+    gx, gy, gz = rd("grid_xAxis.csv")[0], rd("grid_yAxis.csv")[0], rd("grid_zAxis.csv")[0]
+    dem = bsar.Dem(rd("DEM_X_vec.csv")[0], rd("DEM_Y_vec.csv")[0], rd("DEM_Z_matrix.csv"))
+    tx, rx = rd("Tx_positions.csv"), rd("Rx_positions.csv")
+    p = bsar.Params(float(g["fc"]), float(g["c"]), float(g["fs"]), float(g["n1"]), float(g["n2"]), int(g["upsample"]))
+    for line in SERVE_JOBS.splitlines():
+        if not line or line.startswith("#"):
+            continue
+        tag, ix0, ix1, iy0, iy1, iz0, iz1, K, prec = line.split()
+        ix0, ix1, iy0, iy1, iz0, iz1, K = map(int, (ix0, ix1, iy0, iy1, iz0, iz1, K))
+        K = K or len(tx)
+        ds = ctx.dataset(p, dem, tx[:K], rx[:K])
+        ds.load_rootdata(g["root"][:K])
+        want, _ = ds.backproject(gx[ix0:ix1], gy[iy0:iy1], gz[iz0:iz1], bsar.MIXED if prec == "mixed" else bsar.FP64)
+        ds.close()
+        got = parse_image((tmp_path / f"{tag}_3D_matrix.bin").read_bytes())
+        assert got.shape == want.shape
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), tag
+        if tag == "full":
+            ref = parse_image(g["out_bin"].tobytes())
+            assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
