@@ -40,7 +40,7 @@ void usage()
 {
     std::printf(
         "usage: bsar_serve [--dir DIR] [--jobs FILE] [--fc HZ] [--fs HZ] [--c M_S] [--factor K] [--device N]\n"
-        "                  [--round-ms MS] [--max-jobs N] [--check]\n"
+        "                  [--round-ms MS] [--max-jobs N] [--hbm-cap-gb GB] [--check]\n"
         "  job lines: <tag> <ix0> <ix1> <iy0> <iy1> <iz0> <iz1> <pulses> <fp64|mixed>\n"
         "  DIR defaults to build/output/proc; jobs are read from standard input without --jobs.\n"
         "  --check parses the inputs and the job list and prints them without using the GPU.\n");
@@ -110,7 +110,7 @@ double quantile(std::vector<double> v, double q)
 int main(int argc, char** argv)
 {
     std::string dir = "build/output/proc", jobs_path;
-    double fc = 10e9, fs = 60e6, c = 3e8, round_ms = 0.0;
+    double fc = 10e9, fs = 60e6, c = 3e8, round_ms = 0.0, cap_gb = 0.0;
     int factor = 4, max_jobs = 0;
     bool check_only = false;
     for (int i = 1; i < argc; ++i) {
@@ -128,6 +128,7 @@ int main(int argc, char** argv)
         else if (a == "--device") setenv("BSAR_DEVICE", next("--device").c_str(), 1);
         else if (a == "--round-ms") round_ms = std::stod(next("--round-ms"));
         else if (a == "--max-jobs") max_jobs = std::stoi(next("--max-jobs"));
+        else if (a == "--hbm-cap-gb") cap_gb = std::stod(next("--hbm-cap-gb"));
         else if (a == "--check") check_only = true;
         else if (a == "--help" || a == "-h") { usage(); return 0; }
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 2; }
@@ -164,6 +165,7 @@ int main(int argc, char** argv)
         bsar_server_config cfg{};
         cfg.max_jobs = max_jobs;
         cfg.round_seconds = round_ms > 0 ? round_ms * 1e-3 : 0.0;
+        cfg.hbm_cap_bytes = cap_gb > 0 ? std::max<int64_t>(1, (int64_t)(cap_gb * 1e9)) : 0;
         bsar_server* srv = nullptr;
         int st = bsar_server_create(dev ? std::atoi(dev) : 0, &cfg, &srv);
         if (st != BSAR_OK) {

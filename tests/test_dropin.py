@@ -214,3 +214,15 @@ def test_serve_images_match_backproject(tmp_path, bsar, ctx):
         if tag == "full":
             ref = parse_image(g["out_bin"].tobytes())
             assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_serve_job_over_capacity_exits_1(tmp_path):
+    """A job whose working memory alone exceeds the server's HBM cap is
+    rejected at submission (BSAR_ERR_CAPACITY); the tool reports it and
+    exits 1 after the other jobs."""
+    g = np.load(GOLDEN / "ref_bp_random.npz")
+    write_proc_dir(tmp_path, g)
+    r = run_serve("--dir", tmp_path, "--hbm-cap-gb", "1e-9", stdin=SERVE_JOBS)
+    assert r.returncode == 1
+    assert "[ERROR] job full: submit status" in r.stderr
