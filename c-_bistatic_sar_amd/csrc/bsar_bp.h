@@ -55,7 +55,7 @@ __device__ __forceinline__ double r1_mean_fp64(const double* __restrict__ ax, co
 // Plain FP64 body, proc_test_script.cpp:484-668 in the reference's operation
 // order (built with -ffp-contract=off); the R1 means take faithful square
 // roots (r1_mean_fp64).  Both bodies run the search with the side stop when
-// the DEM is in LDS (kSplit).
+// the DEM is in LDS (kSplit); this one without the probe reuse (kReuse).
 template <bool kSplit = false, bool kPairs = false>
 __device__ __forceinline__ void bp_fp64_thread(const DemDev& d, const BpArgs& a, int g, int64_t v, int p0, int p1)
 {
@@ -70,8 +70,8 @@ __device__ __forceinline__ void bp_fp64_thread(const DemDev& d, const BpArgs& a,
 
     for (int i = p0; i < p1; ++i) {
         double qax, qay, qaz, qbx, qby, qbz;
-        fermat_solve<kSplit, kPairs, kSplit>(d, a.traj.tx[i], a.traj.ty[i], a.traj.tz[i], Px, Py, Pz, n1, n2, qax, qay, qaz);   // rdp:166-172
-        fermat_solve<kSplit, kPairs, kSplit>(d, Px, Py, Pz, a.traj.rx[i], a.traj.ry[i], a.traj.rz[i], n2, n1, qbx, qby, qbz);   // rdp:175-181
+        fermat_solve<kSplit, kPairs, kSplit, false>(d, a.traj.tx[i], a.traj.ty[i], a.traj.tz[i], Px, Py, Pz, n1, n2, qax, qay, qaz);   // rdp:166-172
+        fermat_solve<kSplit, kPairs, kSplit, false>(d, Px, Py, Pz, a.traj.rx[i], a.traj.ry[i], a.traj.rz[i], n2, n1, qbx, qby, qbz);   // rdp:175-181
         double e0 = Px - qax, e1 = Py - qay, e2 = Pz - qaz;
         const double R2t = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:496-505
         e0 = Px - qbx; e1 = Py - qby; e2 = Pz - qbz;

@@ -332,7 +332,7 @@ __device__ __forceinline__ double exact_opt(const DemDev& d, const Leg& L, bool 
     return L.n1 * d1 + L.n2 * d2;
 }
 
-template <int G, bool kSplit, bool kPairs, bool kSide>
+template <int G, bool kSplit, bool kPairs, bool kSide, bool kReuse>
 __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, double R, double& bestX, double& bestY,
                                              double& minOpt)
 {
@@ -353,19 +353,30 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     const float g1f = (float)g1sq, g2f = (float)g2sq;
 
     // Probe: the centre candidate (the previous level's best) evaluated
-    // exactly first, so the bounds below start from a tight value.
+    // exactly first, so the bounds below start from a tight value.  When its
+    // grid coordinates round to exactly the previous best's (about half the
+    // time), its value is already known: minOpt was computed at those very
+    // coordinates by the same expression (minOpt < 1e9: a previous level set
+    // it; the first level's centre is the unevaluated midpoint).  A tie with
+    // minOpt never updates the result (strict '<' across levels), so starting
+    // the level from it is the same as evaluating it.  kReuse: on except in
+    // the plain FP64 back-projection kernel, which measured faster without it.
     double lvMin, M;
     int lvK = C * G + C;
     {
-        const double px = minX + C * spanX / (G - 1);
-        int i0 = 0, di = 0;
-        double wx = 0.0;
-        if (!empty) axis_lookup(d.X, d.nx, d.x0, d.ihx, xsorted, px, i0, di, wx);
-        const double dx1 = px - L.x1, dx2 = L.x2 - px;
-        lvMin = exact_opt(d, L, empty, ysorted, i0, di, wx, dx1 * dx1, dx2 * dx2, minY + C * spanY / (G - 1));
-        if (!(lvMin == lvMin)) {          // a NaN path never wins (strict '<' in the reference)
-            lvMin = __builtin_inf();
-            lvK = 0x7fffffff;
+        const double px = minX + C * spanX / (G - 1), py = minY + C * spanY / (G - 1);
+        if (kReuse && px == bestX && py == bestY && minOpt < 1e9) {
+            lvMin = minOpt;
+        } else {
+            int i0 = 0, di = 0;
+            double wx = 0.0;
+            if (!empty) axis_lookup(d.X, d.nx, d.x0, d.ihx, xsorted, px, i0, di, wx);
+            const double dx1 = px - L.x1, dx2 = L.x2 - px;
+            lvMin = exact_opt(d, L, empty, ysorted, i0, di, wx, dx1 * dx1, dx2 * dx2, py);
+            if (!(lvMin == lvMin)) {          // a NaN path never wins (strict '<' in the reference)
+                lvMin = __builtin_inf();
+                lvK = 0x7fffffff;
+            }
         }
         M = L.screen ? fmin(minOpt, lvMin) : __builtin_inf();
     }
@@ -542,7 +553,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
 }
 
 // calculateRefractionPoint, raw_data_processing.cpp:455-572.
-template <bool kSplit = false, bool kPairs = false, bool kSide = false>
+template <bool kSplit = false, bool kPairs = false, bool kSide = false, bool kReuse = true>
 __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double y1, double z1,
                                              double x2, double y2, double z2, double n1, double n2,
                                              double& qx, double& qy, double& qz)
@@ -553,11 +564,11 @@ __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double 
     double R = std_max(__builtin_sqrt(ex * ex + ey * ey), 500.0);    // :483-484
     double minOpt = 1e9;                                             // :497
     const Leg L = make_leg(d, x1, y1, z1, x2, y2, z2, n1, n2);
-    fermat_level<15, kSplit, kPairs, kSide>(d, L, R, bestX, bestY, minOpt);
+    fermat_level<15, kSplit, kPairs, kSide, kReuse>(d, L, R, bestX, bestY, minOpt);
     R /= 5.0;                                                        // :551
 #pragma unroll 1
     for (int level = 1; level < 5; ++level) {
-        fermat_level<11, kSplit, kPairs, kSide>(d, L, R, bestX, bestY, minOpt);
+        fermat_level<11, kSplit, kPairs, kSide, kReuse>(d, L, R, bestX, bestY, minOpt);
         R /= 3.0;
     }
     qx = bestX;                                                      // :555-557
