@@ -7,7 +7,7 @@
 // formats.  Radar constants default to the reference's compiled-in values
 // (radar_init.cpp:9,17,44,74): fc = 10 GHz, fs = 60 MHz, c = 3e8, factor 4
 // (proc_test_script.cpp:220).  Exit status 1 on any load error
-// (proc_test_script.cpp:898-901).
+// (proc_test_script.cpp:898-901), 2 on a malformed command line.
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -43,11 +43,15 @@ int main(int argc, char** argv)
             return argv[++i];
         };
         if (a == "--dir") dir = next("--dir");
-        else if (a == "--fc") fc = std::stod(next("--fc"));
-        else if (a == "--fs") fs = std::stod(next("--fs"));
-        else if (a == "--c") c = std::stod(next("--c"));
-        else if (a == "--factor") factor = std::stoi(next("--factor"));
-        else if (a == "--precision") precision = next("--precision") == "mixed" ? BSAR_MIXED : BSAR_FP64;
+        else if (a == "--fc") fc = option_number(next("--fc"), "--fc");
+        else if (a == "--fs") fs = option_number(next("--fs"), "--fs");
+        else if (a == "--c") c = option_number(next("--c"), "--c");
+        else if (a == "--factor") factor = option_integer(next("--factor"), "--factor");
+        else if (a == "--precision") {
+            const std::string p = next("--precision");
+            if (p != "fp64" && p != "mixed") { std::fprintf(stderr, "--precision must be fp64 or mixed\n"); return 2; }
+            precision = p == "mixed" ? BSAR_MIXED : BSAR_FP64;
+        }
         else if (a == "--device") setenv("BSAR_DEVICE", next("--device").c_str(), 1);
         else if (a == "--check") check_only = true;
         else if (a == "--help" || a == "-h") { usage(); return 0; }

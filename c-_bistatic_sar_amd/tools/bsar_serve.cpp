@@ -17,7 +17,9 @@
 // Exit status: 0 when every job completed, 1 on a load or job error, 2 on a
 // malformed command line or job list.
 #include <algorithm>
+#include <cctype>
 #include <chrono>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -121,14 +123,14 @@ int main(int argc, char** argv)
         };
         if (a == "--dir") dir = next("--dir");
         else if (a == "--jobs") jobs_path = next("--jobs");
-        else if (a == "--fc") fc = std::stod(next("--fc"));
-        else if (a == "--fs") fs = std::stod(next("--fs"));
-        else if (a == "--c") c = std::stod(next("--c"));
-        else if (a == "--factor") factor = std::stoi(next("--factor"));
+        else if (a == "--fc") fc = option_number(next("--fc"), "--fc");
+        else if (a == "--fs") fs = option_number(next("--fs"), "--fs");
+        else if (a == "--c") c = option_number(next("--c"), "--c");
+        else if (a == "--factor") factor = option_integer(next("--factor"), "--factor");
         else if (a == "--device") setenv("BSAR_DEVICE", next("--device").c_str(), 1);
-        else if (a == "--round-ms") round_ms = std::stod(next("--round-ms"));
-        else if (a == "--max-jobs") max_jobs = std::stoi(next("--max-jobs"));
-        else if (a == "--hbm-cap-gb") cap_gb = std::stod(next("--hbm-cap-gb"));
+        else if (a == "--round-ms") round_ms = option_number(next("--round-ms"), "--round-ms");
+        else if (a == "--max-jobs") max_jobs = option_integer(next("--max-jobs"), "--max-jobs");
+        else if (a == "--hbm-cap-gb") cap_gb = option_number(next("--hbm-cap-gb"), "--hbm-cap-gb");
         else if (a == "--check") check_only = true;
         else if (a == "--help" || a == "-h") { usage(); return 0; }
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); usage(); return 2; }
@@ -208,9 +210,14 @@ int main(int argc, char** argv)
                 continue;
             }
             const std::string out = dir + "/" + j.tag + "_3D_matrix.bin";
-            fa::check(bsar_write_image(out.c_str(), reinterpret_cast<const double*>(j.img.data()), (int)j.gx.size(),
-                                       (int)j.gy.size(), (int)j.gz.size(), j.gx.data(), j.gy.data(), j.gz.data()),
-                      nullptr, "export");
+            const int ws = bsar_write_image(out.c_str(), reinterpret_cast<const double*>(j.img.data()),
+                                            (int)j.gx.size(), (int)j.gy.size(), (int)j.gz.size(), j.gx.data(),
+                                            j.gy.data(), j.gz.data());
+            if (ws != BSAR_OK) {
+                std::fprintf(stderr, "[ERROR] job %s: cannot write %s (status %d)\n", j.tag.c_str(), out.c_str(), ws);
+                ++failed;
+                continue;
+            }
             const double ms = (info.t_done - info.t_submit) * 1e3;
             lat.push_back(ms);
             t_first = std::min(t_first, info.t_submit);

@@ -7,6 +7,7 @@
 #pragma once
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <sstream>
 #include <string>
@@ -18,6 +19,34 @@
 namespace bsar_tools {
 
 namespace fa = bsar::facade;
+
+// Command-line values: the whole string must be a number (an integer for
+// option_integer); otherwise the tool prints the option and exits with 2.
+inline double option_number(const std::string& v, const char* what)
+{
+    size_t used = 0;
+    double x = 0.0;
+    try {
+        x = std::stod(v, &used);
+    } catch (const std::exception&) {
+        used = 0;
+    }
+    if (used == 0 || used != v.size()) {
+        std::fprintf(stderr, "bad value '%s' for %s\n", v.c_str(), what);
+        std::exit(2);
+    }
+    return x;
+}
+
+inline int option_integer(const std::string& v, const char* what)
+{
+    const double x = option_number(v, what);
+    if (!(x >= -2147483648.0 && x <= 2147483647.0) || x != (double)(long long)x) {
+        std::fprintf(stderr, "bad value '%s' for %s\n", v.c_str(), what);
+        std::exit(2);
+    }
+    return (int)x;
+}
 
 struct Dem {
     std::vector<double> X_vec, Y_vec;

@@ -174,6 +174,28 @@ def test_serve_rejects_malformed_jobs(tmp_path, line):
     assert r.returncode == 2 and "[ERROR]" in r.stderr
 
 
+@pytest.mark.parametrize("args", [("--fc", "abc"), ("--fs", "1e9x"), ("--factor", "2.5"), ("--factor", "9e99"),
+                                  ("--c", ""), ("--precision", "fp32")])
+def test_tools_reject_malformed_options(tmp_path, args):
+    """A bad option value exits with 2 and names the option, before any file is read."""
+    g = np.load(GOLDEN / "ref_bp_random.npz")
+    write_proc_dir(tmp_path, g)
+    tools = [run_tool] if args[0] == "--precision" else [run_tool, run_serve]
+    for tool in tools:
+        r = tool("--check", "--dir", tmp_path, *args) if tool is run_tool else \
+            tool("--check", "--dir", tmp_path, *args, stdin=SERVE_JOBS)
+        assert r.returncode == 2, (tool.__name__, r.returncode, r.stderr)
+        assert args[0] in r.stderr
+
+
+def test_serve_rejects_malformed_numeric_options(tmp_path):
+    g = np.load(GOLDEN / "ref_bp_random.npz")
+    write_proc_dir(tmp_path, g)
+    for args in (("--round-ms", "fast"), ("--max-jobs", "1.5"), ("--hbm-cap-gb", "lots")):
+        r = run_serve("--check", "--dir", tmp_path, *args, stdin=SERVE_JOBS)
+        assert r.returncode == 2 and args[0] in r.stderr
+
+
 def test_serve_missing_input_exits_1(tmp_path):
     g = np.load(GOLDEN / "ref_bp_random.npz")
     write_proc_dir(tmp_path, g)
