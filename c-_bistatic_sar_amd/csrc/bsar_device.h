@@ -399,7 +399,8 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     // dropped this way could not win or tie, so the argmin is unchanged.
     // kSide: on in the back-projection kernels with the DEM in LDS, where the
     // saved bound evaluations pay; the search-only kernels, at 3 waves per
-    // SIMD, would spill the extra state.
+    // SIMD, would spill the extra state.  kSide also ends a row's screening
+    // loop early (below), which measured slower in the search-only kernels.
     const float slope = kSlope * (L.n1f + L.n2f);
     unsigned cdone = 0u;
     auto part = [&](auto jb, auto nj) {
@@ -492,8 +493,15 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
             const float2* zp = d.Zp + i0;
             const float wxf = di ? (float)wx : 0.0f, dxaf = (float)dxa, dxbf = (float)dxb;
             float Mf = (float)(M * (1.0 + 0x1p-22));   // >= M: an upper bound of the minimum
+            // Columns past the row's last kept one (centre-out order) have
+            // +inf (or NaN) estimates that neither lower Mf nor pass colok, so
+            // with kSide the loop stops at a chunk of four columns once no
+            // lane of the wave keeps one in it; loads within a chunk still
+            // issue together.
+            const int nact = 32 - __builtin_clz(colok);   // colok != 0
 #pragma unroll
             for (int t = 0; t < NJ; ++t) {
+                if (kSide && t > 0 && (t & 3) == 0 && nact <= t) break;
                 float za, zb;
                 if (kPairs) {
                     // (z, dz/dx) pairs: one 8-byte LDS load per row of the cell
