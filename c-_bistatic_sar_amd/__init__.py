@@ -106,6 +106,8 @@ SIGNATURES = {
     "bsar_range_compress": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, _dp]),
     "bsar_range_compress_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_int,
                                              C.POINTER(ReportC)]),
+    "bsar_range_compress_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _dp, C.c_int,
+                                                     C.c_int, C.POINTER(ReportC)]),
     "bsar_chirp": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
     "bsar_dataset_create": (C.c_int, [C.c_void_p, C.POINTER(ParamsC), C.POINTER(DemC), _dp, _dp, C.c_int,
                                       C.POINTER(C.c_void_p)]),
@@ -287,9 +289,11 @@ class Context:
         return out
 
     def range_compress_device(self, buf: "DeviceBuffer", S: int, N: int, ref, conj: bool = True,
-                              precision: int = FP64) -> dict:
+                              precision: int = FP64, noalloc: bool = False) -> dict:
         """In place on a device buffer of [N][S] complex128 (FP64) / complex64
-        (MIXED); returns the run report (device time of the filter)."""
+        (MIXED); returns the run report (device time of the filter).  noalloc
+        calls bsar_range_compress_device_noalloc (the library's fixed per-GPU
+        scratch instead of device allocations)."""
         esz = 16 if precision == FP64 else 8
         if buf.nbytes < S * N * esz:
             raise ValueError("device buffer too small")
@@ -297,8 +301,9 @@ class Context:
         if len(r) != S:
             raise ValueError("reference length must equal the column length")
         rep = ReportC()
-        self._chk(self.L.bsar_range_compress_device(self.h, C.c_void_p(buf.ptr), S, N, _ptr(r.view(np.float64)),
-                                                    int(conj), precision, C.byref(rep)), "range_compress_device")
+        fn = self.L.bsar_range_compress_device_noalloc if noalloc else self.L.bsar_range_compress_device
+        self._chk(fn(self.h, C.c_void_p(buf.ptr), S, N, _ptr(r.view(np.float64)), int(conj), precision,
+                     C.byref(rep)), "range_compress_device_noalloc" if noalloc else "range_compress_device")
         return {f: getattr(rep, f) for f, _ in ReportC._fields_}
 
     def dataset(self, params: Params, dem: Dem, tx, rx) -> "Dataset":

@@ -302,6 +302,9 @@ int bsar_ctx_create(int device, bsar_ctx** out)
         bsar_ctx_destroy(c);
         return BSAR_ERR_DEVICE;
     }
+    // device address of this GPU's no-allocation scratch (once per device);
+    // a failed lookup only disables bsar_range_compress_device_noalloc
+    (void)bsar::scratch_init(device);
     *out = c;
     return BSAR_OK;
 }

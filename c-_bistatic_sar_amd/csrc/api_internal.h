@@ -53,6 +53,10 @@ extern "C" void bsar_r1tab_free(bsar_dataset* ds);
 namespace bsar {
 
 int set_error(bsar_ctx* ctx, int status, const std::string& msg);
+
+// Looks up, once per device and with that device current, the device address
+// of the no-allocation entry point's file-scope scratch array (rc.cpp).
+int scratch_init(int device);
 int hip_fail(bsar_ctx* ctx, hipError_t e, const char* what);
 
 // Builds the device DEM description from the caller's arrays (validation,

@@ -23,6 +23,16 @@ inline size_t fft_filter_tmp_elems(size_t S, size_t ncols, size_t elem_bytes)
 }
 hipError_t fft_filter(double2* data, double2* tmp, const double2* H, int S, int64_t ncols, hipStream_t st);
 hipError_t fft_filter(float2* data, float2* tmp, const float2* H, int S, int64_t ncols, hipStream_t st);
+// Same with the twiddle tables supplied: tw of length S / 2 and twn of length
+// S / 4 (both entries of at least one), as fft_twiddles fills them; no
+// allocation on any path.
+hipError_t fft_filter_tw(double2* data, double2* tmp, const double2* H, int S, int64_t ncols, hipStream_t st,
+                         const double2* tw, const double2* twn);
+hipError_t fft_filter_tw(float2* data, float2* tmp, const float2* H, int S, int64_t ncols, hipStream_t st,
+                         const float2* tw, const float2* twn);
+// tw[k] = exp(-2 pi i k / S), k < max(S / 2, 1) -- the cached tables' contents.
+hipError_t fft_twiddles(double2* tw, int S, hipStream_t st);
+hipError_t fft_twiddles(float2* tw, int S, hipStream_t st);
 // Forward FFT of ncols columns (global passes); *result is data or tmp.
 hipError_t fft_forward(double2* data, double2* tmp, int S, int64_t ncols, hipStream_t st, double2** result);
 hipError_t to_f32(const double2* a, float2* b, int64_t n, hipStream_t st);

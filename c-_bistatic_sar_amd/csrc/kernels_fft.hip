@@ -451,9 +451,12 @@ static hipError_t twiddle_table(int S, hipStream_t st, const typename C2<T>::typ
     return hipSuccess;
 }
 
+// tw_given / twn_given: twiddle tables of S and S / 2 the caller built
+// (fft_twiddles); null takes the cached per-process tables.
 template <typename T>
 hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, const typename C2<T>::type* H, int S,
-                        int64_t ncols, hipStream_t st)
+                        int64_t ncols, hipStream_t st, const typename C2<T>::type* tw_given = nullptr,
+                        const typename C2<T>::type* twn_given = nullptr)
 {
     using V = typename C2<T>::type;
     if (S < 2 || (S & (S - 1)) || ncols <= 0) return hipErrorInvalidValue;
@@ -478,8 +481,8 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
     if ((size_t)S * sizeof(V) <= kFftLdsMax) {
         // S / 2^KP <= 1024 threads: S <= 16384 (complex64) / 8192 (complex128)
         if (!tmp) return hipErrorInvalidValue;
-        const V* tw = nullptr;
-        hipError_t e = twiddle_table<T>(S, st, &tw);
+        const V* tw = tw_given;
+        hipError_t e = tw ? hipSuccess : twiddle_table<T>(S, st, &tw);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_bitrev_copy<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS);
         if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -489,9 +492,9 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
         // two half-length LDS filters between a DIF split and a DIT merge
         if (!tmp) return hipErrorInvalidValue;
         const int n = S / 2;
-        const V *tw = nullptr, *twn = nullptr;
-        hipError_t e = twiddle_table<T>(S, st, &tw);
-        if (e == hipSuccess) e = twiddle_table<T>(n, st, &twn);
+        const V *tw = tw_given, *twn = twn_given;
+        hipError_t e = tw ? hipSuccess : twiddle_table<T>(S, st, &tw);
+        if (e == hipSuccess && !twn) e = twiddle_table<T>(n, st, &twn);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_split_spectrum<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS - 1);
         hipLaunchKernelGGL(k_split_dif<T>, dim3(nblk(ncols * n, 256)), dim3(256), 0, st, data, tw, n, ncols);
@@ -528,6 +531,31 @@ hipError_t fft_filter(float2* data, float2* tmp, const float2* H, int S, int64_t
 {
     return fft_filter_t<float>(data, tmp, H, S, ncols, st);
 }
+
+hipError_t fft_filter_tw(double2* data, double2* tmp, const double2* H, int S, int64_t ncols, hipStream_t st,
+                         const double2* tw, const double2* twn)
+{
+    if (!tw || !twn) return hipErrorInvalidValue;
+    return fft_filter_t<double>(data, tmp, H, S, ncols, st, tw, twn);
+}
+
+hipError_t fft_filter_tw(float2* data, float2* tmp, const float2* H, int S, int64_t ncols, hipStream_t st,
+                         const float2* tw, const float2* twn)
+{
+    if (!tw || !twn) return hipErrorInvalidValue;
+    return fft_filter_t<float>(data, tmp, H, S, ncols, st, tw, twn);
+}
+
+template <typename T>
+static hipError_t twiddles_into(typename C2<T>::type* tw, int S, hipStream_t st)
+{
+    const int n = S / 2 > 0 ? S / 2 : 1;
+    hipLaunchKernelGGL(k_twiddles<T>, dim3(nblk(n, 256)), dim3(256), 0, st, tw, S);
+    return hipGetLastError();
+}
+
+hipError_t fft_twiddles(double2* tw, int S, hipStream_t st) { return twiddles_into<double>(tw, S, st); }
+hipError_t fft_twiddles(float2* tw, int S, hipStream_t st) { return twiddles_into<float>(tw, S, st); }
 
 hipError_t fft_forward(double2* data, double2* tmp, int S, int64_t ncols, hipStream_t st, double2** result)
 {

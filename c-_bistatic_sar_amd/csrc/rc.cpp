@@ -2,6 +2,8 @@
 // pulse columns against one reference (fft_utils.cpp:63-137 as driven by
 // raw_data_processing.cpp:406-446).
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "api_internal.h"
@@ -123,6 +125,134 @@ extern "C" int bsar_range_compress_device(bsar_ctx* c, void* d_sig, int S, int N
         rep->seconds = ms * 1e-3;
         rep->pulses = N;
         rep->voxels = 0;
+        rep->launches = 1;
+    }
+    return BSAR_OK;
+}
+
+// ---- range compression with no allocation --------------------------------
+//
+// bsar_range_compress_device_noalloc runs bsar_range_compress_device's kernels
+// with every buffer they need beyond the caller's columns (the reference
+// spectrum, its FFT ping-pong buffer, the filter's bit-reversed spectrum or
+// global-pass buffer, the FP32 spectrum, the twiddle tables) carved out of
+// one file-scope device array.  Each GPU holds its own copy of `scratch`; its
+// device address is looked up once per device with that device current
+// (scratch_init, called when a context is created) and only that address is
+// handed to kernels and copies.  Every call on a GPU shares the GPU's array:
+// calls that overlap on one GPU (two host threads, two contexts) race.
+
+namespace {
+
+constexpr size_t kScratchBytes = size_t(4) << 20;   // 4 MiB per GPU
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+size_t up256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct ScratchLayout {
+    size_t ref, tmp1, tmp, h32, tw, twn, total;
+};
+
+ScratchLayout scratch_layout(size_t S, size_t N, size_t esz, bool mixed)
+{
+    ScratchLayout L{};
+    size_t off = 0;
+    L.ref = off;  off += up256(S * sizeof(double2));
+    L.tmp1 = off; off += up256(S * sizeof(double2));
+    L.tmp = off;  off += up256(fft_filter_tmp_elems(S, N, esz) * esz);
+    L.h32 = off;  off += mixed ? up256(S * sizeof(float2)) : 0;
+    L.tw = off;   off += up256((S / 2 > 0 ? S / 2 : 1) * esz);
+    L.twn = off;  off += up256((S / 4 > 0 ? S / 4 : 1) * esz);
+    L.total = off;
+    return L;
+}
+
+}  // namespace
+
+namespace bsar {
+
+int scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return BSAR_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_done[device]) return g_scratch_err[device] == hipSuccess ? BSAR_OK : BSAR_ERR_DEVICE;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_done[device] = true;
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    return e == hipSuccess ? BSAR_OK : BSAR_ERR_DEVICE;
+}
+
+}  // namespace bsar
+
+extern "C" int bsar_range_compress_device_noalloc(bsar_ctx* c, void* d_sig, int S, int N, const double* ref,
+                                                  int conj, int precision, bsar_report* rep)
+{
+    if (!c) return BSAR_ERR_INVALID;
+    if (S < 2 || (S & (S - 1)) || N < 0 || (N && !d_sig) || !ref)
+        return set_error(c, BSAR_ERR_INVALID,
+                         "range_compress_device_noalloc needs a power-of-two S >= 2 and buffers");
+    if (precision != BSAR_FP64 && precision != BSAR_MIXED) return set_error(c, BSAR_ERR_INVALID, "bad precision");
+    if (S > (1 << 26)) return set_error(c, BSAR_ERR_INVALID, "column too long");
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    const bool mixed = precision == BSAR_MIXED;
+    const size_t P = (size_t)S, esz = mixed ? sizeof(float2) : sizeof(double2);
+    const ScratchLayout L = scratch_layout(P, (size_t)N, esz, mixed);
+    if (L.total > kScratchBytes)
+        return set_error(c, BSAR_ERR_CAPACITY,
+                         "range_compress_device_noalloc: S = " + std::to_string(S) + ", N = " + std::to_string(N) +
+                             " needs " + std::to_string(L.total) + " bytes of scratch; the no-allocation entry "
+                             "point has " + std::to_string(kScratchBytes) + " (use bsar_range_compress_device)");
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c->device >= 0 && c->device < kMaxDevices && g_scratch_done[c->device])
+            base = g_scratch_ptr[c->device];
+    }
+    if (!base) return set_error(c, BSAR_ERR_DEVICE, "no device scratch on this GPU (address lookup failed)");
+    BSAR_CHECK_HIP(c, hipSetDevice(c->device));
+    if (N == 0) return BSAR_OK;
+    unsigned char* b = static_cast<unsigned char*>(base);
+    double2* dref = reinterpret_cast<double2*>(b + L.ref);
+    double2* dtmp1 = reinterpret_cast<double2*>(b + L.tmp1);
+    BSAR_CHECK_HIP(c, hipMemcpyAsync(dref, ref, P * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    double2* H = nullptr;
+    BSAR_CHECK_HIP(c, fft_forward(dref, dtmp1, (int)P, 1, c->stream, &H));
+    if (conj) BSAR_CHECK_HIP(c, conj_inplace(H, (int64_t)P, c->stream));
+    if (mixed) {
+        float2* h32 = reinterpret_cast<float2*>(b + L.h32);
+        float2* tw = reinterpret_cast<float2*>(b + L.tw);
+        float2* twn = reinterpret_cast<float2*>(b + L.twn);
+        BSAR_CHECK_HIP(c, to_f32(H, h32, (int64_t)P, c->stream));
+        BSAR_CHECK_HIP(c, fft_twiddles(tw, (int)P, c->stream));
+        BSAR_CHECK_HIP(c, fft_twiddles(twn, (int)(P / 2), c->stream));
+        BSAR_CHECK_HIP(c, hipEventRecord(c->ev0, c->stream));
+        BSAR_CHECK_HIP(c, fft_filter_tw((float2*)d_sig, reinterpret_cast<float2*>(b + L.tmp), h32, (int)P, N,
+                                        c->stream, tw, twn));
+    } else {
+        double2* tw = reinterpret_cast<double2*>(b + L.tw);
+        double2* twn = reinterpret_cast<double2*>(b + L.twn);
+        BSAR_CHECK_HIP(c, fft_twiddles(tw, (int)P, c->stream));
+        BSAR_CHECK_HIP(c, fft_twiddles(twn, (int)(P / 2), c->stream));
+        BSAR_CHECK_HIP(c, hipEventRecord(c->ev0, c->stream));
+        BSAR_CHECK_HIP(c, fft_filter_tw((double2*)d_sig, reinterpret_cast<double2*>(b + L.tmp), H, (int)P, N,
+                                        c->stream, tw, twn));
+    }
+    BSAR_CHECK_HIP(c, hipEventRecord(c->ev1, c->stream));
+    BSAR_CHECK_HIP(c, hipEventSynchronize(c->ev1));
+    if (rep) {
+        float ms = 0.f;
+        BSAR_CHECK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        rep->seconds = ms * 1e-3;
+        rep->pulses = N;
         rep->launches = 1;
     }
     return BSAR_OK;

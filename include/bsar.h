@@ -164,6 +164,19 @@ BSAR_API int bsar_range_compress(bsar_ctx* ctx, const double* sig, int S, int N,
 BSAR_API int bsar_range_compress_device(bsar_ctx* ctx, void* d_sig, int S, int N, const double* ref, int conj,
                                int precision, bsar_report* rep);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): bsar_range_compress_device with every buffer beyond the
+ * caller's columns (reference spectrum, FFT buffers, twiddle tables) taken
+ * from a fixed 4 MiB scratch array per GPU instead of hipMalloc.  Same
+ * kernels, same bits.  Sizes whose buffers exceed the scratch return
+ * BSAR_ERR_CAPACITY: up to S = 16384 (complex128) / 32768 (complex64) every N
+ * fits (about 60 S bytes); longer columns take N * S elements more.  All calls
+ * on one GPU share its scratch: calls that overlap on one GPU (two host
+ * threads or contexts) race.  The call returns after its kernels end. */
+BSAR_API int bsar_range_compress_device_noalloc(bsar_ctx* ctx, void* d_sig, int S, int N, const double* ref,
+                                                int conj, int precision, bsar_report* rep);
+
 /* createCHIRP, raw_data_processing.cpp:257-290: writes L = int(T*fs)+1
  * samples when out != NULL; returns L (> 0) or a negative status. */
 BSAR_API int bsar_chirp(double f0, double f1, double T, double fs, double* out);
