@@ -10,7 +10,13 @@ on the same host.
     python bench.py --workload wide
 
 One step = one full mixed-precision image of the workload with the
-range-compressed data already resident in HBM.  Prints one JSON line on
+range-compressed data already resident in HBM.  After the timed steps the
+default run adds: the plain FP64 image (and PSLR/ISLR of the mixed one
+against it), parity of both images against the oracle on a strided voxel
+subset, the reference's own OpenMP proc_test_script at the workload's radar
+constants on a 16^3 strided subset (the CPU baseline, whose image is
+compared with the GPU's), and the 256-job serving stream (mixed, plus FP64 on
+its first 16 jobs) with p50/p99 job latency.  Prints one JSON line on
 stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -67,56 +73,91 @@ def load(name: str, path: Path, pkg: bool = False):
     return mod
 
 
-def roofline(rep: dict, n: int, form: str, traffic) -> dict:
+PROFILE_FLOPS = "valu_flops_bp_large.md"
+PROFILE_TRAFFIC = "traffic_bp_large.md"
+PROFILE_ISSUE = "valu_bp_large.md"
+
+
+def profile_table(name: str) -> tuple[dict, str | None]:
+    """Rows of the first markdown table in profiles/<name> keyed by the first
+    cell (each row a dict by header), and the commit the profile states
+    ("commit: <sha>" line); ({}, None) when the file is absent."""
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return {}, None
+    rows, head, commit = {}, None, None
+    for line in p.read_text().splitlines():
+        m = re.match(r"\s*commit:\s*`?([0-9a-f]{7,40})`?", line)
+        if m:
+            commit = m.group(1)
+        if not line.strip().startswith("|"):
+            if head is not None and rows:
+                break
+            continue
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if head is None:
+            head = cells
+        elif set("".join(cells)) <= set("-: "):
+            continue
+        else:
+            rows[cells[0]] = dict(zip(head, cells))
+    return rows, commit
+
+
+def _num(row: dict, key: str):
+    try:
+        return float(row[key].replace(",", ""))
+    except (KeyError, ValueError, AttributeError):
+        return None
+
+
+def roofline(rep: dict, n: int, form: str) -> dict:
+    """Roofline of the dominant back-projection kernel of one image.
+
+    achieved = executed FP64 + FP32 flop of one launch / the launch's average
+    device time (HIP events on the context's stream, measured in this run).
+    The executed flop per voxel-pulse comes from rocprofv3 counters
+    (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F64,F32}: 64 lanes per wave
+    instruction, FMA = 2) over the same image, committed in
+    profiles/valu_flops_bp_large.md.  peak = 78.6 TFLOP/s: MI355X's FP64 vector
+    peak, which is also the FP32 ceiling of this build (packed FP32
+    instructions are disabled, so FP32 issues 64 FMA lanes per CU per clock,
+    as FP64 does).  effective_tflops counts the reference algorithm's flop
+    instead (709 exact candidates per refraction solve), which the screened
+    search does not execute."""
     f64, f32 = (flops_mixed if form == "mixed" else flops_fp64)(n)
     launches = max(int(rep["launches"]) - 1, 1) if rep["launches"] > 1 else 1
     t_launch = rep["seconds"] / launches
     vp_launch = rep["voxel_pulses"] / launches
-    eq = (f64 + f32 * P64 / P32) * vp_launch          # FP64-equivalent flop per launch
-    ach = eq / t_launch
     kernel = "k_bp_mixed" if form == "mixed" else "k_bp_fp64"
-    return {"bound": "valu", "achieved": round(ach / 1e12, 3), "peak": P64 / 1e12, "unit": "TFLOP/s",
-            "frac": round(ach / P64, 4), "traffic": traffic,
-            "kernel": kernel,
-            "valu_issue_util": issue_from_profiles(kernel),
-            "note": "achieved counts the reference algorithm's flop (709 exact candidates per refraction solve); the "
-                    "screened search estimates ~160 candidates in FP32 and evaluates ~10 exactly, so frac can exceed "
-                    "1 -- hardware efficiency is valu_issue_util (rocprofv3 counters, profiles/valu_bp_large.md)",
-            "avg_launch_ms": round(t_launch * 1e3, 3), "launches": launches,
-            "flop_per_voxel_pulse": {"fp64": f64, "fp32": f32}}
-
-
-def traffic_from_profiles(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE passes, calibrated against a copy
-    kernel of known size); None when it is absent."""
-    p = ROOT / "profiles" / "traffic_bp_large.md"
-    if not p.exists():
-        return None
-    for line in p.read_text().splitlines():
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == kernel:
-            try:
-                return int(cells[4])
-            except ValueError:
-                return None
-    return None
-
-
-def issue_from_profiles(kernel: str):
-    """VALU issue utilisation of `kernel` (wave-instructions issued / the
-    chip's peak issue rate) from the committed rocprofv3 --pmc summary."""
-    p = ROOT / "profiles" / "valu_bp_large.md"
-    if not p.exists():
-        return None
-    for line in p.read_text().splitlines():
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 7 and cells[0] == kernel:
-            try:
-                return float(cells[6])
-            except ValueError:
-                return None
-    return None
+    flops, fcommit = profile_table(PROFILE_FLOPS)
+    traffic, tcommit = profile_table(PROFILE_TRAFFIC)
+    issue, icommit = profile_table(PROFILE_ISSUE)
+    row = flops.get(kernel, {})
+    e64, e32 = _num(row, "fp64 flop/vp"), _num(row, "fp32 flop/vp")
+    out = {"bound": "valu", "achieved": None, "peak": P64 / 1e12, "unit": "TFLOP/s", "frac": None,
+           "traffic": None, "kernel": kernel, "avg_launch_ms": round(t_launch * 1e3, 3), "launches": launches,
+           "voxel_pulses_per_launch": vp_launch}
+    if e64 is not None and e32 is not None:
+        ach = (e64 + e32) * vp_launch / t_launch
+        out.update(achieved=round(ach / 1e12, 3), frac=round(ach / P64, 4),
+                   executed_flop_per_voxel_pulse={"fp64": e64, "fp32": e32,
+                                                  "source": f"profiles/{PROFILE_FLOPS}",
+                                                  "commit": fcommit})
+    tb = _num(traffic.get(kernel, {}), "bytes per launch")
+    if tb is not None:
+        out["traffic"] = int(tb)
+        out["traffic_source"] = {"source": f"profiles/{PROFILE_TRAFFIC}", "commit": tcommit,
+                                 "algorithmic_bytes_per_launch": _num(traffic[kernel], "algorithmic bytes")}
+    iu = _num(issue.get(kernel, {}), "issue util")
+    if iu is not None:
+        out["valu_issue_util"] = {"value": iu, "source": f"profiles/{PROFILE_ISSUE}", "commit": icommit}
+    eff = (f64 + f32) * vp_launch / t_launch
+    out["effective_tflops"] = round(eff / 1e12, 3)
+    out["reference_flop_per_voxel_pulse"] = {"fp64": f64, "fp32": f32}
+    out["peak_note"] = ("FP64 vector peak; also the FP32 ceiling here (no packed FP32: 64 FMA lanes per CU per "
+                        "clock in both precisions)")
+    return out
 
 
 def cpu_model() -> str:
@@ -146,66 +187,120 @@ def cpu_port_all_cores(sc, root, side: int = 10) -> dict:
             "sample": f"{side}x{side}x{side} strided voxels x {sc.n} pulses in {secs:.1f} s"}
 
 
-def cpu_baseline(sc, root, target_s: float = 20.0) -> dict | None:
-    """The reference's own OpenMP back-projection (oracle/_ref/proc_test_script,
-    built from the reference sources) on a strided voxel subset of the same
-    workload; falls back to the C restatement (oracle/liboracle.so)."""
-    ref = ROOT / "oracle" / "_ref" / "proc_test_script"
-    # about 15 s of the reference's 12-thread voxel loop at 2048 pulses
-    side = 12 if sc.n >= 1024 else 16
-    nx, ny, nz = side, side, side
-    sub = lambda a, m: a[np.linspace(0, len(a) - 1, m).round().astype(int)]
-    gx, gy, gz = sub(sc.gx, nx), sub(sc.gy, ny), sub(sc.gz, nz)
-    vox = nx * ny * nz
+def strided(a, m):
+    """m indices spread evenly over a (both ends included)."""
+    return np.linspace(0, len(a) - 1, m).round().astype(int)
+
+
+def cpu_baseline(sc, root, side: int = 16) -> tuple[dict, dict | None]:
+    """The reference's own OpenMP back-projection on a strided side^3 voxel
+    subset of the same workload: oracle/_ref/proc_test_script_env, the
+    reference's proc_test_script built from its sources with the radar
+    constants read from the environment, so it runs at this workload's
+    carrier, sampling rate and speed of light on the same file set (written
+    in the reference's formats).  Its timer covers the voxel loop
+    (proc_test_script.cpp:448-801), the reflection-point search included.
+    Returns the baseline and the reference's image of the subset (axis indices
+    and values) for the parity check; falls back to the C restatement
+    (oracle/liboracle.so, image None) where the reference was not built."""
+    ref = ROOT / "oracle" / "_ref" / "proc_test_script_env"
+    ix, iy, iz = strided(sc.gx, side), strided(sc.gy, side), strided(sc.gz, side)
+    gx, gy, gz = sc.gx[ix], sc.gy[iy], sc.gz[iz]
+    vox = side ** 3
     cores = os.cpu_count() or 1
     if ref.exists():
         work = Path(tempfile.mkdtemp(prefix="bsar_cpu_"))
         try:
-            load("bsar_amd.scene", PKG_DIR / "scene.py").write_reference_inputs(
-                work / "build" / "output" / "proc", sc, root, gx, gy, gz)
+            proc = work / "build" / "output" / "proc"
+            load("bsar_amd.scene", PKG_DIR / "scene.py").write_reference_inputs(proc, sc, root, gx, gy, gz)
             threads = min(int(os.environ.get("OMP_NUM_THREADS", cores)), cores)
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), BSAR_REF_FC=repr(sc.fc), BSAR_REF_FS=repr(sc.fs),
+                       BSAR_REF_C=repr(sc.c))
             t0 = time.time()
-            r = subprocess.run([str(ref)], cwd=work, capture_output=True, text=True, env=env, timeout=900)
+            r = subprocess.run([str(ref)], cwd=work, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                               env=env, timeout=900)
             wall = time.time() - t0
-            if r.returncode == 0:
-                m = re.search(r"Procesamiento paralelo completado en: (\d+) ms", r.stdout)
-                if m:
-                    secs = int(m.group(1)) / 1e3
-                    used = min(threads, 12)   # proc_test_script.cpp:428 caps the team at 12
-                    return {"value": round(vox * sc.n / secs, 1), "unit": "voxel-pulse/s", "cores": used,
-                            "kind": "reference",
-                            "sample": f"{vox} voxels ({nx}x{ny}x{nz} strided subset) x {sc.n} pulses; "
-                                      f"voxel loop {secs:.1f} s (wall {wall:.1f} s with file load + x4 spline); "
-                                      "reference constants fc=10 GHz, fs=60 MHz x4 compiled in, same work per "
-                                      "voxel-pulse; logging to a pipe"}
+            m = re.search(r"Procesamiento paralelo completado en: (\d+) ms", r.stdout) if r.returncode == 0 else None
+            if m:
+                secs = int(m.group(1)) / 1e3
+                used = min(threads, 12)   # proc_test_script.cpp:428 caps the team at 12
+                blob = (proc / "output_3D_matrix.bin").read_bytes()
+                img = parse_image(blob)
+                return ({"value": round(vox * sc.n / secs, 1), "unit": "voxel-pulse/s", "cores": used,
+                         "kind": "reference",
+                         "sample": f"{vox} voxels ({side}x{side}x{side} strided subset of the grid) x {sc.n} pulses "
+                                   f"of the same data; the reference's proc_test_script at fc={sc.fc:g} Hz, "
+                                   f"fs={sc.fs:g} Hz x4, c={sc.c:g} m/s; voxel loop {secs:.1f} s by its own timer "
+                                   f"(wall {wall:.1f} s with file load + x4 spline); its logging to a pipe"},
+                        {"ix": ix, "iy": iy, "iz": iz, "img": img})
             log(f"reference CPU run failed (rc={r.returncode}); using the C restatement")
         finally:
             shutil.rmtree(work, ignore_errors=True)
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
-    up = O.upsample(root[:, :], sc.upsample)
-    ids = np.arange(vox)
+    up = O.upsample(root, sc.upsample)
     t0 = time.time()
-    O.backproject(sc.bp_params(), sc.dem, sc.tx, sc.rx, up, gx, gy, gz, subset=ids, reflection=True, threads=cores)
+    O.backproject(sc.bp_params(), sc.dem, sc.tx, sc.rx, up, gx, gy, gz, reflection=True, threads=cores)
     secs = time.time() - t0
-    return {"value": round(vox * sc.n / secs, 1), "unit": "voxel-pulse/s", "cores": cores, "kind": "port",
-            "sample": f"{vox} voxels x {sc.n} pulses, C restatement with the reflection search"}
+    return ({"value": round(vox * sc.n / secs, 1), "unit": "voxel-pulse/s", "cores": cores, "kind": "port",
+             "sample": f"{vox} voxels x {sc.n} pulses, C restatement with the reflection search"}, None)
 
 
-def run_serve(args, pkg, scene) -> dict:
+def parse_image(blob: bytes) -> np.ndarray:
+    """output_3D_matrix.bin (export3DComplexMatrix, data_export.cpp:180-309):
+    52-byte header, dims (nx, ny, nz), complex128 payload x fastest."""
+    import struct
+    nx, ny, nz = struct.unpack_from("<QQQ", blob, 28)
+    return np.frombuffer(blob[52:], np.complex128).reshape(nz, ny, nx)
+
+
+def oracle_parity(sc, root, img64, imgm) -> dict:
+    """The FP64 and mixed images against the oracle's back-projection
+    (oracle/liboracle.so, the C restatement of proc_test_script.cpp:451-798)
+    on a strided 4x4x4 voxel subset plus the voxel nearest each unit point
+    target (where the image peaks), relative to the FP64 image peak."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    nx, ny, nz = len(sc.gx), len(sc.gy), len(sc.gz)
+    IX, IY, IZ = (a.ravel() for a in np.meshgrid(strided(sc.gx, 4), strided(sc.gy, 4), strided(sc.gz, 4),
+                                                  indexing="ij"))
+    pts = sc.scat[np.abs(sc.rcs) >= 0.999]
+    near = lambda a, v: int(np.abs(a - v).argmin())
+    ix = np.concatenate([IX, [near(sc.gx, p[0]) for p in pts]])
+    iy = np.concatenate([IY, [near(sc.gy, p[1]) for p in pts]])
+    iz = np.concatenate([IZ, [near(sc.gz, p[2]) for p in pts]])
+    ids = np.unique(iz * nx * ny + ix * ny + iy)         # the oracle's z -> x -> y voxel order
+    t0 = time.time()
+    want = O.backproject(sc.bp_params(), sc.dem, sc.tx, sc.rx, O.upsample(root, sc.upsample), sc.gx, sc.gy, sc.gz,
+                         subset=ids)
+    secs = time.time() - t0
+    yi, xi, zi = ids % ny, (ids // ny) % nx, ids // (nx * ny)
+    w = want[zi, yi, xi]
+    peak = float(np.abs(img64).max())
+    e64 = float(np.abs(img64[zi, yi, xi] - w).max()) / peak
+    em = float(np.abs(imgm[zi, yi, xi] - w).max()) / peak
+    return {"voxels": int(len(ids)), "subset": "4x4x4 strided + the voxel nearest each of "
+                                               f"{len(pts)} point targets", "oracle_s": round(secs, 2),
+            "fp64_max_err_rel_peak": e64, "mixed_err_db": round(20 * np.log10(max(em, 1e-300)), 2),
+            "subset_peak_rel": round(float(np.abs(w).max()) / peak, 4)}
+
+
+def serve_stream(pkg, scene, n_jobs: int = 256, rate: float = 40.0, round_ms: float = 40.0,
+                 fp64_jobs: int | None = None) -> dict:
     """Serving stream: jobs arrive (Poisson) and run through one bsar_server
-    with continuous batching; the same stream runs once in mixed precision and
-    once in plain FP64.  Latency = submission -> image in the caller's host
-    buffer (server clock); throughput = completed voxel-pulses / makespan."""
+    with continuous batching, once in mixed precision and, for the first
+    fp64_jobs jobs of the same stream (None: all, 0: none), in plain FP64.
+    Latency = submission -> image in the caller's host buffer (server clock);
+    throughput = completed voxel-pulses / makespan."""
     import threading
     sc = scene.make_scene("serve")
-    jobs = scene.serve_jobs(args.jobs, args.rate)
+    jobs = scene.serve_jobs(n_jobs, rate)
     total_vp = sum(j["vp"] for j in jobs)
-    log(f"serving stream: {len(jobs)} jobs, Poisson {args.rate}/s over {jobs[-1]['t']:.2f} s, "
+    log(f"serving stream: {len(jobs)} jobs, Poisson {rate}/s over {jobs[-1]['t']:.2f} s, "
         f"{total_vp:.3e} voxel-pulses; dataset {sc.n} pulses x {sc.S} samples")
-    samples = [k for k, j in enumerate(jobs) if np.prod(j["shape"]) <= 64 ** 3][:6]
-    forms = [("mixed", pkg.MIXED)] + ([] if args.no_fp64 else [("fp64", pkg.FP64)])
+    n64 = len(jobs) if fp64_jobs is None else min(fp64_jobs, len(jobs))
+    samples = [k for k, j in enumerate(jobs[:max(n64, 1)]) if np.prod(j["shape"]) <= 64 ** 3][:6]
+    forms = [("mixed", pkg.MIXED, jobs)] + ([("fp64", pkg.FP64, jobs[:n64])] if n64 > 0 else [])
     res, imgs = {}, {}
     with pkg.Context(0) as ctx:
         t0 = time.time()
@@ -215,13 +310,13 @@ def run_serve(args, pkg, scene) -> dict:
         ds.load_rootdata(root)
         del root
         log(f"dataset resident: {ds.device_bytes() / 1e9:.2f} GB")
-        for name, prec in forms:
-            with pkg.Server(round_seconds=args.round_ms * 1e-3) as srv:
-                ids = [None] * len(jobs)
+        for name, prec, stream in forms:
+            with pkg.Server(round_seconds=round_ms * 1e-3) as srv:
+                ids = [None] * len(stream)
                 t_start = time.perf_counter()
 
                 def feed():
-                    for k, j in enumerate(jobs):
+                    for k, j in enumerate(stream):
                         dt = t_start + j["t"] - time.perf_counter()
                         if dt > 0:
                             time.sleep(dt)
@@ -230,13 +325,13 @@ def run_serve(args, pkg, scene) -> dict:
                 th = threading.Thread(target=feed)
                 th.start()
                 infos = []
-                while len(infos) < len(jobs):
+                while len(infos) < len(stream):
                     try:
                         jid, img, info = srv.wait_any(timeout=1.0)
                     except TimeoutError:
                         continue
                     except pkg.BsarError as e:
-                        if e.status == -1 and len(infos) < len(jobs):   # nothing submitted yet
+                        if e.status == -1 and len(infos) < len(stream):   # nothing submitted yet
                             time.sleep(0.01)
                             continue
                         raise
@@ -244,17 +339,17 @@ def run_serve(args, pkg, scene) -> dict:
                     k = ids.index(jid)
                     if k in samples:
                         imgs[(name, k)] = img
-                    j = jobs[k]
-                    log(f"{name}: job {k} ({'x'.join(map(str, j['shape']))} x {j['K']}) done in "
-                        f"{info['latency']:.2f} s; {len(infos)}/{len(jobs)} done at "
-                        f"{time.perf_counter() - t_start:.1f} s")
+                    if len(infos) % 32 == 0 or len(infos) == len(stream):
+                        log(f"{name}: {len(infos)}/{len(stream)} jobs done at {time.perf_counter() - t_start:.1f} s")
                 th.join()
                 st = srv.stats()
             t_sub = min(i["t_submit"] for i in infos)
             t_end = max(i["t_done"] for i in infos)
             span = t_end - t_sub
             lat = np.array([i["latency"] for i in infos])
-            res[name] = {"value": round(total_vp / span, 1), "makespan_s": round(span, 3),
+            vp = sum(j["vp"] for j in stream)
+            res[name] = {"value": round(vp / span, 1), "jobs": len(stream), "voxel_pulses": vp,
+                         "makespan_s": round(span, 3),
                          "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 1),
                                         "p99": round(float(np.percentile(lat, 99)) * 1e3, 1),
                                         "mean": round(float(lat.mean()) * 1e3, 1)},
@@ -264,23 +359,32 @@ def run_serve(args, pkg, scene) -> dict:
             log(f"{name}: {res[name]['value']:.3e} voxel-pulse/s, p50 {res[name]['latency_ms']['p50']} ms, "
                 f"p99 {res[name]['latency_ms']['p99']} ms, device busy {res[name]['device_busy']}")
         ds.close()
-    m = res["mixed"]
+    out = {"workload": f"serve: {len(jobs)} jobs, Poisson {rate}/s, grids {scene.SERVE_GRIDS}, pulses "
+                       f"{scene.SERVE_PULSES} (first K of {sc.n} x {sc.S} samples x4)",
+           "jobs": len(jobs), "rate": rate, "round_ms": round_ms, "mixed": res["mixed"]}
+    if "fp64" in res:
+        out["fp64"] = res["fp64"]
+        if n64 < len(jobs):
+            out["fp64"]["note"] = f"the first {n64} jobs of the stream"
+        errs = [float(np.abs(imgs[("mixed", k)] - imgs[("fp64", k)]).max() / np.abs(imgs[("fp64", k)]).max())
+                for k in samples]
+        out["mixed_vs_fp64"] = {"throughput_ratio": round(res["mixed"]["value"] / res["fp64"]["value"], 3),
+                                "peak_err_db_max": round(20 * np.log10(max(max(errs), 1e-300)), 2),
+                                "jobs_compared": len(samples)}
+    return out
+
+
+def run_serve(args, pkg, scene) -> dict:
+    srv = serve_stream(pkg, scene, args.jobs, args.rate, args.round_ms, 0 if args.no_fp64 else None)
+    m = srv["mixed"]
     out = {"metric": "serving stream: aggregate voxel-pulse BP/s (mixed precision) with p50/p99 job latency",
            "value": m["value"], "unit": "voxel-pulse/s", "n_gpus": 1, "steps": 1, "warmup": 0,
            "ms_per_step": round(m["makespan_s"] * 1e3, 1), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic (seeded scene and job stream)",
-           "config": {"workload": f"serve: {len(jobs)} jobs, Poisson {args.rate}/s, grids {scene.SERVE_GRIDS}, "
-                                  f"pulses {scene.SERVE_PULSES} (first K of {sc.n} x {sc.S} samples x4)",
-                      "jobs": len(jobs), "rate": args.rate, "voxel_pulses": total_vp,
-                      "round_ms": args.round_ms},
-           "latency_ms": m["latency_ms"], "mixed": m}
-    if "fp64" in res:
-        out["fp64"] = res["fp64"]
-        errs = [float(np.abs(imgs[("mixed", k)] - imgs[("fp64", k)]).max() / np.abs(imgs[("fp64", k)]).max())
-                for k in samples]
-        out["mixed_vs_fp64"] = {"throughput_ratio": round(m["value"] / res["fp64"]["value"], 3),
-                                "peak_err_db_max": round(20 * np.log10(max(max(errs), 1e-300)), 2),
-                                "jobs_compared": len(samples)}
+           "config": {"workload": srv["workload"], "jobs": srv["jobs"], "rate": srv["rate"],
+                      "voxel_pulses": m["voxel_pulses"], "round_ms": srv["round_ms"]},
+           "latency_ms": m["latency_ms"]}
+    out.update({k: v for k, v in srv.items() if k in ("mixed", "fp64", "mixed_vs_fp64")})
     return out
 
 
@@ -450,7 +554,7 @@ def run_wide(args, pkg, scene) -> dict:
                                   f"x{sc.upsample}, 64 scatterers, B 500 MHz", "voxels": V, "pulses": N,
                       "samples": S},
            "wall_s": round(wall_m, 2),
-           "roofline": roofline(rep, N, "mixed", None),
+           "roofline": roofline(rep, N, "mixed"),
            "range_compression": {"fp32_ms": round(rc32["seconds"] * 1e3, 3), "fp64_ms": round(rc64["seconds"] * 1e3, 3),
                                  "fp32_gb_s": round(gbs, 1), "roofline_frac": round(gbs * 1e9 / 8e12, 4),
                                  "algorithmic_bytes": bytes32, "bound": "hbm",
@@ -478,6 +582,10 @@ def main() -> int:
     ap.add_argument("--round-ms", type=float, default=40.0, help="serve: target device time per round")
     ap.add_argument("--chunk", type=int, default=65536, help="rough: voxels per refraction launch")
     ap.add_argument("--check-solves", type=int, default=100000, help="rough: solves checked against the CPU")
+    ap.add_argument("--no-serve", action="store_true", help="large/small: skip the serving stream")
+    ap.add_argument("--no-parity", action="store_true", help="large/small: skip the oracle parity check")
+    ap.add_argument("--serve-fp64-jobs", type=int, default=16,
+                    help="large/small: FP64 serving on this prefix of the stream (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -529,8 +637,11 @@ def main() -> int:
                                       f"{sc.S} samples x{sc.upsample}", "voxels": V, "pulses": N,
                           "samples": sc.S, "precision": "mixed"},
                "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 1),
-                              "p99": round(float(np.percentile(lat, 99)) * 1e3, 1)}}
-        out["roofline"] = roofline(reps[-1], N, "mixed", traffic_from_profiles("k_bp_mixed"))
+                              "p99": round(float(np.percentile(lat, 99)) * 1e3, 1),
+                              "note": "whole-image step time; job latency of the serving stream is under serving"},
+               "device_s_per_step": round(float(np.mean([r["seconds"] for r in reps])), 4)}
+        out["roofline"] = roofline(reps[-1], N, "mixed")
+        img64 = None
         if not args.no_fp64:
             t1 = time.perf_counter()
             img64, rep64 = ds.backproject(sc.gx, sc.gy, sc.gz, pkg.FP64)
@@ -538,7 +649,7 @@ def main() -> int:
             log(f"fp64 image: {w64:.2f} s wall, {rep64['seconds']:.2f} s device")
             err = float(np.abs(img_m - img64).max() / np.abs(img64).max())
             out["fp64"] = {"value": round(V * N / w64, 1), "ms": round(w64 * 1e3, 1),
-                           "roofline": roofline(rep64, N, "fp64", traffic_from_profiles("k_bp_fp64"))}
+                           "device_s": round(rep64["seconds"], 4), "roofline": roofline(rep64, N, "fp64")}
             out["mixed_vs_fp64"] = {"speedup": round(w64 / (elapsed / args.steps), 3),
                                     "peak_err_db": round(20 * np.log10(max(err, 1e-300)), 2)}
             Q = load("bsar_amd.quality", PKG_DIR / "quality.py")
@@ -552,16 +663,39 @@ def main() -> int:
             log(f"PSLR/ISLR deltas over {lob['targets']} point targets: max |dPSLR| "
                 f"{lob['max_abs_d_pslr_db']:.2e} dB, max |dISLR| {lob['max_abs_d_islr_db']:.2e} dB")
         ds.close()
+    if img64 is not None and not args.no_parity:
+        par = oracle_parity(sc, root, img64, img_m)
+        par["tolerance"] = {"fp64": 1e-9, "mixed_db": -80.0}
+        out["parity"] = par
+        log(f"parity vs oracle on {par['voxels']} voxels: fp64 {par['fp64_max_err_rel_peak']:.2e} of the peak, "
+            f"mixed {par['mixed_err_db']} dB")
     if not args.no_cpu:
-        log("cpu baseline ...")
-        cb = cpu_baseline(sc, root)
+        log("cpu baseline (reference proc_test_script at the workload's constants) ...")
+        cb, refimg = cpu_baseline(sc, root)
         out["cpu_baseline"] = cb
-        if cb:
-            cb["cpu_model"] = cpu_model()
-            out["vs_cpu"] = round(value / cb["value"], 1)
-            log(f"cpu baseline: {cb['value']:.3e} voxel-pulse/s on {cb['cores']} threads ({cb['kind']}); "
-                "C restatement on every core ...")
-            cb["all_cores"] = cpu_port_all_cores(sc, root)
+        cb["cpu_model"] = cpu_model()
+        out["vs_cpu"] = round(value / cb["value"], 1)
+        log(f"cpu baseline: {cb['value']:.3e} voxel-pulse/s on {cb['cores']} threads ({cb['kind']})")
+        if refimg is not None and img64 is not None and "parity" in out:
+            sub = img64[np.ix_(refimg["iz"], refimg["iy"], refimg["ix"])]
+            subm = img_m[np.ix_(refimg["iz"], refimg["iy"], refimg["ix"])]
+            peak = float(np.abs(img64).max())
+            e = float(np.abs(sub - refimg["img"]).max()) / peak
+            em = float(np.abs(subm - refimg["img"]).max()) / peak
+            out["parity"]["fp64_vs_reference"] = {
+                "voxels": int(refimg["img"].size), "max_err_rel_peak": e,
+                "mixed_err_db": round(20 * np.log10(max(em, 1e-300)), 2),
+                "subset_peak_rel": round(float(np.abs(refimg["img"]).max()) / peak, 4),
+                "binary": "oracle/_ref/proc_test_script_env (the reference's proc_test_script, radar constants "
+                          "from the environment)"}
+            log(f"fp64 vs the reference's own image on {refimg['img'].size} voxels: {e:.2e} of the peak")
+        log("C restatement on every core ...")
+        cb["all_cores"] = cpu_port_all_cores(sc, root)
+    if not args.no_serve:
+        srv = serve_stream(pkg, scene, args.jobs, args.rate, args.round_ms, args.serve_fp64_jobs)
+        out["serving"] = srv
+        out["latency_ms"]["serving_p50"] = srv["mixed"]["latency_ms"]["p50"]
+        out["latency_ms"]["serving_p99"] = srv["mixed"]["latency_ms"]["p99"]
     print(json.dumps(out), flush=True)
     return 0
 

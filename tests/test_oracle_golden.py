@@ -112,3 +112,42 @@ def test_backprojection_vs_reference_binary(oracle, tag):
     assert np.abs(ref).max() > 0
     err = np.abs(img - ref).max() / np.abs(ref).max()
     assert err <= 1e-14, err
+
+
+REF_ENV = GOLDEN.parents[1] / "oracle" / "_ref" / "proc_test_script_env"
+
+
+def _run_ref_env(tmp_path, sc, root, env_extra):
+    import os
+    import subprocess
+    sc_mod = __import__("conftest").load_scene()
+    sc_mod.write_reference_inputs(tmp_path / "build" / "output" / "proc", sc, root)
+    env = dict(os.environ, OMP_NUM_THREADS="2", **env_extra)
+    for k in ("BSAR_REF_FC", "BSAR_REF_FS", "BSAR_REF_C"):
+        if k not in env_extra:
+            env.pop(k, None)
+    r = subprocess.run([str(REF_ENV)], cwd=tmp_path, capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:]
+    return parse_image((tmp_path / "build" / "output" / "proc" / "output_3D_matrix.bin").read_bytes())
+
+
+@pytest.mark.skipif(not REF_ENV.exists(), reason="the compiled reference (oracle/_ref) is not built")
+def test_reference_driver_at_workload_constants(tmp_path, oracle, scene_mod):
+    """The reference's own proc_test_script, linked with its radar constants
+    read from the environment (oracle/ref_radar_env.cpp), images a workload
+    file set at the workload's carrier and sampling rate (fc = 1 GHz,
+    fs = 800 MHz x 4): its output_3D_matrix.bin matches the oracle's
+    back-projection with those constants; with the variables unset it gives
+    the compiled-in constants' image (10 GHz, 60 MHz)."""
+    sc = scene_mod.make_scene("small", n_pulses=24, grid=(5, 4, 3), samples=1024)
+    _, _, root = oracle.simulate(sc.sim_params(), sc.dem, sc.tx, sc.rx, sc.scat, sc.rcs)
+    root = np.ascontiguousarray(root.T)
+    ref = _run_ref_env(tmp_path / "a", sc, root, {"BSAR_REF_FC": "1e9", "BSAR_REF_FS": "800e6", "BSAR_REF_C": "3e8"})
+    up = oracle.upsample(root, 4)
+    want = oracle.backproject(sc.bp_params(), sc.dem, sc.tx, sc.rx, up, sc.gx, sc.gy, sc.gz, threads=2)
+    assert np.abs(want).max() > 0
+    assert np.abs(ref - want).max() <= 1e-14 * np.abs(want).max()
+    dflt = _run_ref_env(tmp_path / "b", sc, root, {})
+    bp = dict(sc.bp_params(), fs4=60e6 * 4, lam=3e8 / 10e9)
+    want = oracle.backproject(bp, sc.dem, sc.tx, sc.rx, up, sc.gx, sc.gy, sc.gz, threads=2)
+    assert np.abs(dflt - want).max() <= 1e-14 * np.abs(want).max()
