@@ -108,6 +108,7 @@ SIGNATURES = {
                                              C.POINTER(ReportC)]),
     "bsar_range_compress_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _dp, C.c_int,
                                                      C.c_int, C.POINTER(ReportC)]),
+    "bsar_fft": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int]),
     "bsar_chirp": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
     "bsar_dataset_create": (C.c_int, [C.c_void_p, C.POINTER(ParamsC), C.POINTER(DemC), _dp, _dp, C.c_int,
                                       C.POINTER(C.c_void_p)]),
@@ -287,6 +288,16 @@ class Context:
         self._chk(self.L.bsar_range_compress(self.h, _ptr(s.view(np.float64)), S, N, _ptr(r.view(np.float64)),
                                              int(conj), precision, _ptr(out.view(np.float64))), "range_compress")
         return out
+
+    def fft(self, data, inverse: bool = False) -> np.ndarray:
+        """FFTProcessor::fft / ::ifft of each row of data ([N][S] or [S]
+        complex); returns the transformed copy."""
+        d = _c128(data).copy()
+        S = d.shape[-1] if d.ndim else 1
+        N = d.size // S if S else 0
+        ptr = _ptr(d.view(np.float64)) if d.size else None
+        self._chk(self.L.bsar_fft(self.h, ptr, S, N, int(inverse)), "fft")
+        return d
 
     def range_compress_device(self, buf: "DeviceBuffer", S: int, N: int, ref, conj: bool = True,
                               precision: int = FP64, noalloc: bool = False) -> dict:

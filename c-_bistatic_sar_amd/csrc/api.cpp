@@ -314,6 +314,8 @@ void bsar_ctx_destroy(bsar_ctx* c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : {(void*)c->dem_cache.dX, (void*)c->dem_cache.dY, (void*)c->dem_cache.dZ, c->in_buf, c->out_buf})
+        if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -332,22 +334,73 @@ int bsar_ctx_synchronize(bsar_ctx* c)
 
 double bsar_flops_per_voxel_pulse(int n) { return 65406.0 + 20.0 * (double)n; }
 
+// The context's device copy of `dem`: reused when the arrays hold the same
+// sizes and bytes as the last call's, uploaded (and the old copy freed)
+// otherwise.
+static int ctx_dem(bsar_ctx* c, const bsar_dem* dem, DemDev* out)
+{
+    auto& k = c->dem_cache;
+    if (dem && k.valid && dem->nx == k.nx && dem->ny == k.ny && dem->zrows == k.zrows && dem->zcols == k.zcols) {
+        const size_t nz = (size_t)std::max(dem->zrows, 0) * (size_t)std::max(dem->zcols, 0);
+        const bool same = (!dem->nx || (dem->x && !std::memcmp(dem->x, k.x.data(), k.x.size() * 8))) &&
+                          (!dem->ny || (dem->y && !std::memcmp(dem->y, k.y.data(), k.y.size() * 8))) &&
+                          (!nz || (dem->z && !std::memcmp(dem->z, k.z.data(), nz * 8)));
+        if (same) {
+            *out = k.dem;
+            return BSAR_OK;
+        }
+    }
+    DevBuf bx, by, bz;
+    DemDev d;
+    double zmin, zmax;
+    int st = upload_dem(c, dem, (double**)&bx.p, (double**)&by.p, (double**)&bz.p, &d, &zmin, &zmax);
+    if (st) return st;
+    for (double* p : {k.dX, k.dY, k.dZ})
+        if (p) (void)hipFree(p);
+    k.dX = (double*)bx.p;
+    k.dY = (double*)by.p;
+    k.dZ = (double*)bz.p;
+    bx.p = by.p = bz.p = nullptr;
+    const size_t nz = (size_t)dem->zrows * (size_t)dem->zcols;
+    k.x.assign(dem->x, dem->x + (dem->x ? dem->nx : 0));
+    k.y.assign(dem->y, dem->y + (dem->y ? dem->ny : 0));
+    k.z.assign(dem->z, dem->z + (dem->z ? nz : 0));
+    k.nx = dem->nx;
+    k.ny = dem->ny;
+    k.zrows = dem->zrows;
+    k.zcols = dem->zcols;
+    k.dem = d;
+    k.valid = true;
+    *out = d;
+    return BSAR_OK;
+}
+
+// A context buffer of at least `need` bytes (grow-only).
+static int ctx_buffer(bsar_ctx* c, void** p, size_t* cap, size_t need)
+{
+    if (need <= *cap) return BSAR_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    BSAR_HIP(c, hipMalloc(p, need));
+    *cap = need;
+    return BSAR_OK;
+}
+
 int bsar_dem_interp(bsar_ctx* c, const bsar_dem* dem, const double* xy, int64_t m, double* z)
 {
     if (!c) return BSAR_ERR_INVALID;
     if (m < 0 || (m && (!xy || !z))) return set_error(c, BSAR_ERR_INVALID, "bad point buffer");
     BSAR_HIP(c, hipSetDevice(c->device));
-    DevBuf bx, by, bz, bin, bout;
     DemDev d;
-    double zmin, zmax;
-    int st = upload_dem(c, dem, (double**)&bx.p, (double**)&by.p, (double**)&bz.p, &d, &zmin, &zmax);
+    int st = ctx_dem(c, dem, &d);
     if (st) return st;
     if (m == 0) return BSAR_OK;
-    BSAR_HIP(c, hipMalloc(&bin.p, (size_t)m * 2 * sizeof(double)));
-    BSAR_HIP(c, hipMalloc(&bout.p, (size_t)m * sizeof(double)));
-    BSAR_HIP(c, hipMemcpyAsync(bin.p, xy, (size_t)m * 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    BSAR_HIP(c, launch_dem_interp(d, (const double*)bin.p, m, (double*)bout.p, c->stream));
-    BSAR_HIP(c, hipMemcpyAsync(z, bout.p, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if ((st = ctx_buffer(c, &c->in_buf, &c->in_cap, (size_t)m * 2 * sizeof(double)))) return st;
+    if ((st = ctx_buffer(c, &c->out_buf, &c->out_cap, (size_t)m * sizeof(double)))) return st;
+    BSAR_HIP(c, hipMemcpyAsync(c->in_buf, xy, (size_t)m * 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    BSAR_HIP(c, launch_dem_interp(d, (const double*)c->in_buf, m, (double*)c->out_buf, c->stream));
+    BSAR_HIP(c, hipMemcpyAsync(z, c->out_buf, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BSAR_HIP(c, hipStreamSynchronize(c->stream));
     return BSAR_OK;
 }
@@ -358,17 +411,15 @@ int bsar_refraction_points(bsar_ctx* c, const bsar_dem* dem, const double* legs,
     if (!c) return BSAR_ERR_INVALID;
     if (m < 0 || (m && (!legs || !out))) return set_error(c, BSAR_ERR_INVALID, "bad leg buffer");
     BSAR_HIP(c, hipSetDevice(c->device));
-    DevBuf bx, by, bz, bin, bout;
     DemDev d;
-    double zmin, zmax;
-    int st = upload_dem(c, dem, (double**)&bx.p, (double**)&by.p, (double**)&bz.p, &d, &zmin, &zmax);
+    int st = ctx_dem(c, dem, &d);
     if (st) return st;
     if (m == 0) return BSAR_OK;
-    BSAR_HIP(c, hipMalloc(&bin.p, (size_t)m * 6 * sizeof(double)));
-    BSAR_HIP(c, hipMalloc(&bout.p, (size_t)m * 3 * sizeof(double)));
-    BSAR_HIP(c, hipMemcpyAsync(bin.p, legs, (size_t)m * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    BSAR_HIP(c, launch_refraction(d, (const double*)bin.p, m, n1, n2, (double*)bout.p, c->stream));
-    BSAR_HIP(c, hipMemcpyAsync(out, bout.p, (size_t)m * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if ((st = ctx_buffer(c, &c->in_buf, &c->in_cap, (size_t)m * 6 * sizeof(double)))) return st;
+    if ((st = ctx_buffer(c, &c->out_buf, &c->out_cap, (size_t)m * 3 * sizeof(double)))) return st;
+    BSAR_HIP(c, hipMemcpyAsync(c->in_buf, legs, (size_t)m * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    BSAR_HIP(c, launch_refraction(d, (const double*)c->in_buf, m, n1, n2, (double*)c->out_buf, c->stream));
+    BSAR_HIP(c, hipMemcpyAsync(out, c->out_buf, (size_t)m * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     BSAR_HIP(c, hipStreamSynchronize(c->stream));
     return BSAR_OK;
 }

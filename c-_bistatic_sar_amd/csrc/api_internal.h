@@ -15,6 +15,21 @@ struct bsar_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Last DEM of bsar_dem_interp / bsar_refraction_points, resident on the
+    // device with a host copy of its arrays: a call with the same DEM contents
+    // reuses it (the facade calls these once per voxel).
+    struct {
+        bool valid = false;
+        int nx = 0, ny = 0, zrows = 0, zcols = 0;
+        std::vector<double> x, y, z;
+        double *dX = nullptr, *dY = nullptr, *dZ = nullptr;
+        bsar::DemDev dem{};
+    } dem_cache;
+    // grow-only point / result buffers of the same calls
+    void* in_buf = nullptr;
+    size_t in_cap = 0;
+    void* out_buf = nullptr;
+    size_t out_cap = 0;
 };
 
 struct bsar_dataset {

@@ -38,5 +38,6 @@ hipError_t fft_forward(double2* data, double2* tmp, int S, int64_t ncols, hipStr
 hipError_t to_f32(const double2* a, float2* b, int64_t n, hipStream_t st);
 hipError_t to_f64(const float2* a, double2* b, int64_t n, hipStream_t st);
 hipError_t conj_inplace(double2* a, int64_t n, hipStream_t st);
+hipError_t scale_inplace(double2* a, int64_t n, double s, hipStream_t st);
 
 }  // namespace bsar

@@ -576,6 +576,13 @@ hipError_t to_f64(const float2* a, double2* b, int64_t n, hipStream_t st)
     return hipGetLastError();
 }
 
+hipError_t scale_inplace(double2* a, int64_t n, double s, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale<double>, dim3(nblk(n, 256)), dim3(256), 0, st, a, n, s);
+    return hipGetLastError();
+}
+
 hipError_t conj_inplace(double2* a, int64_t n, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;

@@ -40,6 +40,17 @@ extern "C" int bsar_range_compress(bsar_ctx* c, const double* sig, int S, int N,
     // samples of the circular result are kept (fft_utils.cpp:68-78, 95-98).
     const size_t P = next_pow2((size_t)S);
     if (P > (size_t)1 << 26) return set_error(c, BSAR_ERR_INVALID, "column too long");
+    if (P == 1) {
+        // length-1 transforms are the identity (fft_utils.cpp:14): the
+        // product of the two samples, the reference conjugated for correlation
+        const double hr = ref[0], hi = conj ? -ref[1] : ref[1];
+        for (int k = 0; k < N; ++k) {
+            const double a = sig[2 * k], b = sig[2 * k + 1];
+            out[2 * k] = a * hr - b * hi;
+            out[2 * k + 1] = a * hi + b * hr;
+        }
+        return BSAR_OK;
+    }
     std::vector<double> padded;
     const double* src = sig;
     if (P != (size_t)S) {
@@ -127,6 +138,38 @@ extern "C" int bsar_range_compress_device(bsar_ctx* c, void* d_sig, int S, int N
         rep->voxels = 0;
         rep->launches = 1;
     }
+    return BSAR_OK;
+}
+
+extern "C" int bsar_fft(bsar_ctx* c, double* data, int S, int N, int inverse)
+{
+    if (!c) return BSAR_ERR_INVALID;
+    if (S < 0 || N < 0 || (S && N && !data)) return set_error(c, BSAR_ERR_INVALID, "fft needs S, N >= 0 and a buffer");
+    if (S > (1 << 26)) return set_error(c, BSAR_ERR_INVALID, "column too long");
+    BSAR_CHECK_HIP(c, hipSetDevice(c->device));
+    if (S == 0 || N == 0) return BSAR_OK;
+    const size_t n = (size_t)S * (size_t)N;
+    const bool pow2 = (S & (S - 1)) == 0;
+    // fft_utils.cpp:14-20: lengths 1 and non-powers of two leave the data as
+    // it is; ifft (:46-60) still divides by the length around that no-op.
+    if (S == 1 || !pow2) {
+        if (inverse && S > 1)
+            for (size_t k = 0; k < 2 * n; ++k) data[k] /= (double)S;
+        return BSAR_OK;
+    }
+    Buf a, b;
+    BSAR_CHECK_HIP(c, hipMalloc(&a.p, n * sizeof(double2)));
+    BSAR_CHECK_HIP(c, hipMalloc(&b.p, n * sizeof(double2)));
+    BSAR_CHECK_HIP(c, hipMemcpyAsync(a.p, data, n * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    if (inverse) BSAR_CHECK_HIP(c, conj_inplace((double2*)a.p, (int64_t)n, c->stream));
+    double2* res = nullptr;
+    BSAR_CHECK_HIP(c, fft_forward((double2*)a.p, (double2*)b.p, S, N, c->stream, &res));
+    if (inverse) {
+        BSAR_CHECK_HIP(c, conj_inplace(res, (int64_t)n, c->stream));
+        BSAR_CHECK_HIP(c, scale_inplace(res, (int64_t)n, 1.0 / (double)S, c->stream));
+    }
+    BSAR_CHECK_HIP(c, hipMemcpyAsync(data, res, n * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    BSAR_CHECK_HIP(c, hipStreamSynchronize(c->stream));
     return BSAR_OK;
 }
 

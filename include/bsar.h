@@ -157,6 +157,16 @@ BSAR_API int bsar_upsample(bsar_ctx* ctx, const double* in, int S, int N, int fa
 BSAR_API int bsar_range_compress(bsar_ctx* ctx, const double* sig, int S, int N, const double* ref,
                         int conj, int precision, double* out);
 
+/* FFTProcessor::fft (inverse = 0) and FFTProcessor::ifft (inverse = 1),
+ * fft_utils.cpp:11-60 (declared fft_utils.h:19-29), in place on N columns
+ * of length S: data is [N][S] complex.  Forward: X[k] = sum_n x[n]
+ * exp(-2 pi i n k / S); inverse: conj(FFT(conj(x))) / S.  As in the
+ * reference, S = 1 or S not a power of two leaves the forward transform a
+ * no-op (the reference prints a warning) and the inverse a division by S.
+ * The GPU computes each twiddle directly, where the reference multiplies
+ * them up by recurrence: results agree to ~3e-17 * S of the peak. */
+BSAR_API int bsar_fft(bsar_ctx* ctx, double* data, int S, int N, int inverse);
+
 /* Same filter on a caller-owned device buffer, in place: d_sig is [N][S]
  * complex128 (BSAR_FP64) or complex64 (BSAR_MIXED), S a power of two; ref is
  * the host reference (complex128, length S).  rep (optional) receives the

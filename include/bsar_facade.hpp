@@ -21,6 +21,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <memory>
@@ -28,6 +29,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "bsar.h"
@@ -101,9 +103,47 @@ std::vector<double> calculateRefractionPoint(double x1, double y1, double z1, do
     return out;
 }
 
-// RangeData, raw_data_processing.h:18-23
+// calculateReflectionPoint, raw_data_processing.cpp:575-612 (host: the point
+// is dead for the image): the Tx/Rx horizontal midpoint at the height of the
+// DEM node nearest to it, nodes scanned x-major with the first minimum kept;
+// nodes outside a short Z row are skipped, and no node leaves height 0.
+template <class DEM>
+std::vector<double> calculateReflectionPoint(double txX, double txY, double /*txZ*/, double rxX, double rxY,
+                                             double /*rxZ*/, const DEM& dem)
+{
+    const double mx = (txX + rxX) / 2.0, my = (txY + rxY) / 2.0;
+    double best = 1e9, bz = 0.0;
+    if (!dem.X_vec.empty() && !dem.Y_vec.empty() && !dem.Z_DEM.empty())
+        for (size_t a = 0; a < dem.X_vec.size(); ++a)
+            for (size_t b = 0; b < dem.Y_vec.size() && b < dem.Z_DEM.size(); ++b) {
+                if (a >= dem.Z_DEM[b].size()) continue;
+                const double dx = dem.X_vec[a] - mx, dy = dem.Y_vec[b] - my;
+                const double dd = std::sqrt(dx * dx + dy * dy);
+                if (dd < best) { best = dd; bz = dem.Z_DEM[b][a]; }
+            }
+    return {mx, my, bz};
+}
+
+// RangeData, raw_data_processing.h:18-23.  Converts to any caller type with
+// the same four members (the reference's own RangeData), so
+// `RangeData ranges = calculaSlantRangeFermat(...)` (proc_test_script.cpp:
+// 484-485) compiles with the caller's struct.
 struct RangeData {
     std::vector<double> r1Tx, r2Tx, r1Rx, r2Rx;
+
+    template <class T, class = decltype(std::declval<T&>().r1Tx = std::vector<double>(),
+                                        std::declval<T&>().r2Tx = std::vector<double>(),
+                                        std::declval<T&>().r1Rx = std::vector<double>(),
+                                        std::declval<T&>().r2Rx = std::vector<double>())>
+    operator T() const
+    {
+        T t{};
+        t.r1Tx = r1Tx;
+        t.r2Tx = r2Tx;
+        t.r1Rx = r1Rx;
+        t.r2Rx = r2Rx;
+        return t;
+    }
 };
 
 // calculaSlantRangeFermat, raw_data_processing.cpp:112-255: refraction points
@@ -154,17 +194,7 @@ RangeData calculaSlantRangeFermat(const DEM& dem, const TRAJ& traj, const ENV& e
         r.r2Tx[i] = dist(traj.P.data(), qa);
         r.r1Rx[i] = dist(rx, qb);
         r.r2Rx[i] = dist(traj.P.data(), qb);
-        // reflection point: nearest DEM node to the horizontal midpoint (:575-612)
-        const double mx = (tx[0] + rx[0]) / 2.0, my = (tx[1] + rx[1]) / 2.0;
-        double best = 1e9, bz = 0.0;
-        for (size_t a = 0; a < dem.X_vec.size(); ++a)
-            for (size_t b = 0; b < dem.Y_vec.size(); ++b)
-                if (b < dem.Z_DEM.size() && a < dem.Z_DEM[b].size()) {
-                    const double dx = dem.X_vec[a] - mx, dy = dem.Y_vec[b] - my;
-                    const double dd = std::sqrt(dx * dx + dy * dy);
-                    if (dd < best) { best = dd; bz = dem.Z_DEM[b][a]; }
-                }
-        refl.P_reflection[i] = {mx, my, bz};
+        refl.P_reflection[i] = calculateReflectionPoint(tx[0], tx[1], tx[2], rx[0], rx[1], rx[2], dem);
     }
     return r;
 }
@@ -179,8 +209,16 @@ inline std::vector<std::complex<double>> splineInterpolation(const std::vector<s
     return out;
 }
 
-// FFTProcessor::fftCorrelation / fftConvolution, fft_utils.cpp:63-137
+// FFTProcessor, fft_utils.h:13-84 / fft_utils.cpp:11-195: the transforms run
+// on the GPU (bsar_fft, bsar_range_compress); the helpers are host code with
+// the reference's rules.
 struct FFTProcessor {
+    // fft / ifft, fft_utils.cpp:11-60: in place; a length that is not a power
+    // of two leaves the forward transform a no-op with the reference's warning
+    // on stderr (:17-20), and the inverse a division by the length.
+    static void fft(std::vector<std::complex<double>>& input) { transform(input, 0); }
+    static void ifft(std::vector<std::complex<double>>& input) { transform(input, 1); }
+
     static std::vector<std::complex<double>> fftCorrelation(const std::vector<std::complex<double>>& signal,
                                                             const std::vector<std::complex<double>>& reference)
     {
@@ -192,7 +230,61 @@ struct FFTProcessor {
         return run(signal, reference, 0);
     }
 
+    // multiply, fft_utils.cpp:140-150: element-wise over the shorter length
+    static std::vector<std::complex<double>> multiply(const std::vector<std::complex<double>>& a,
+                                                      const std::vector<std::complex<double>>& b)
+    {
+        const size_t n = a.size() < b.size() ? a.size() : b.size();
+        std::vector<std::complex<double>> r(n);
+        for (size_t i = 0; i < n; ++i) r[i] = a[i] * b[i];
+        return r;
+    }
+
+    // padToPowerOfTwo, fft_utils.cpp:154-161: zero-padded to nextPowerOfTwo
+    static std::vector<std::complex<double>> padToPowerOfTwo(const std::vector<std::complex<double>>& input)
+    {
+        std::vector<std::complex<double>> p(nextPowerOfTwo(input.size()), 0.0);
+        for (size_t i = 0; i < input.size(); ++i) p[i] = input[i];
+        return p;
+    }
+
+    // bitReverse, fft_utils.cpp:164-181: the in-place bit-reversal permutation
+    // (j counts in bit-reversed order beside i, carrying from the top bit of
+    // length / 2 down; pairs with i < j swap), which a length that is not a
+    // power of two runs through the same way
+    static void bitReverse(std::vector<std::complex<double>>& input)
+    {
+        const size_t n = input.size();
+        size_t j = 0;
+        for (size_t i = 1; i < n; ++i) {
+            size_t bit = n >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) std::swap(input[i], input[j]);
+        }
+    }
+
+    // isPowerOfTwo / nextPowerOfTwo, fft_utils.cpp:183-195 (nextPowerOfTwo(0) = 1)
+    static bool isPowerOfTwo(size_t n) { return n > 0 && (n & (n - 1)) == 0; }
+    static size_t nextPowerOfTwo(size_t n)
+    {
+        size_t p = 1;
+        while (p < n) p *= 2;
+        return p;
+    }
+
 private:
+    static void transform(std::vector<std::complex<double>>& x, int inverse)
+    {
+        const size_t n = x.size();
+        if (n > 1 && !isPowerOfTwo(n))
+            std::fprintf(stderr, "Warning: FFT input size %zu is not a power of 2. Consider padding.\n", n);
+        if (n == 0) return;
+        if (n > (size_t)1 << 26) throw std::runtime_error("fft: length above 2^26");
+        bsar_ctx* c = context();
+        check(bsar_fft(c, reinterpret_cast<double*>(x.data()), (int)n, 1, inverse), c, inverse ? "ifft" : "fft");
+    }
+
     static std::vector<std::complex<double>> run(std::vector<std::complex<double>> s,
                                                  std::vector<std::complex<double>> r, int conj)
     {

@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
         L.or_dem_interp.restype = C.c_double
         L.or_dem_interp.argtypes = [C.POINTER(DemO), C.c_double, C.c_double]
         L.or_refraction_point.argtypes = [C.c_double] * 6 + [C.POINTER(DemO), C.c_double, C.c_double, _dp]
+        L.or_reflection_point.argtypes = [C.c_double] * 6 + [C.POINTER(DemO), _dp]
         L.or_slant_range_fermat.argtypes = [C.POINTER(DemO), _dp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp,
                                             _dp, _dp]
         L.or_spline.argtypes = [_dp, C.c_size_t, C.c_int, _dp]
@@ -112,6 +113,19 @@ def refraction_points(dem, legs, n1, n2):
     q = np.empty(3)
     for k, s in enumerate(legs):
         L.or_refraction_point(*s, C.byref(d.s), n1, n2, _p(q))
+        out[k] = q
+    return out
+
+
+def reflection_points(dem, pairs):
+    """calculateReflectionPoint (or_reflection_point) for pairs[m] = (tx, ty, tz, rx, ry, rz)."""
+    d = _DemHold(*dem)
+    L = lib()
+    pairs = _f(pairs).reshape(-1, 6)
+    out = np.empty((len(pairs), 3))
+    q = np.empty(3)
+    for k, s in enumerate(pairs):
+        L.or_reflection_point(*s, C.byref(d.s), _p(q))
         out[k] = q
     return out
 
@@ -266,6 +280,8 @@ def ref_lib():
         L.ref_dem_interp.argtypes = dem_args + [_dp, C.c_int, _dp]
         L.ref_refraction.argtypes = dem_args + [_dp, C.c_int, C.c_double, C.c_double, _dp]
         L.ref_slant_range.argtypes = dem_args + [_dp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.ref_reflection.argtypes = dem_args + [_dp, C.c_int, _dp]
+        L.ref_slant_reflection.argtypes = dem_args + [_dp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp]
         L.ref_spline.argtypes = [_dp, C.c_int, C.c_int, _dp]
         L.ref_fft.argtypes = [_dp, C.c_int]
         L.ref_ifft.argtypes = [_dp, C.c_int]

@@ -92,6 +92,37 @@ void ref_slant_range(const double* x, int nx, const double* y, int ny, const dou
     }
 }
 
+// calculateReflectionPoint over m antenna pairs pr[m][6] = (tx, ty, tz, rx, ry, rz)
+void ref_reflection(const double* x, int nx, const double* y, int ny, const double* z, int zrows, int zcols,
+                    const double* pr, int m, double* out)
+{
+    ProcessedDEM d = make_dem(x, nx, y, ny, z, zrows, zcols);
+    for (int k = 0; k < m; k++) {
+        const double* s = pr + 6 * (size_t)k;
+        std::vector<double> q = calculateReflectionPoint(s[0], s[1], s[2], s[3], s[4], s[5], d);
+        out[3 * k] = q[0]; out[3 * k + 1] = q[1]; out[3 * k + 2] = q[2];
+    }
+}
+
+// the reflection points calculaSlantRangeFermat stores for one target (refl[n][3])
+void ref_slant_reflection(const double* x, int nx, const double* y, int ny, const double* z, int zrows, int zcols,
+                          const double* tx, const double* rx, int n, const double* P, double n1, double n2,
+                          double* refl_out)
+{
+    ProcessedDEM d = make_dem(x, nx, y, ny, z, zrows, zcols);
+    TrajectoryMatrix t;
+    t.Tx.resize(n, std::vector<double>(3));
+    t.Rx.resize(n, std::vector<double>(3));
+    for (int i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) { t.Tx[i][c] = tx[3 * i + c]; t.Rx[i][c] = rx[3 * i + c]; }
+    t.P.assign(P, P + 3);
+    Environment env; env.n1 = n1; env.n2 = n2;
+    ReflectionPoints refl; RefractionPoints refr;
+    calculaSlantRangeFermat(d, t, env, refl, refr, false);
+    for (int i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) refl_out[3 * i + c] = refl.P_reflection[i][c];
+}
+
 void ref_spline(const double* in, int n, int factor, double* out)
 {
     from_cvec(splineInterpolation(to_cvec(in, n), factor), out);

@@ -147,6 +147,30 @@ def kernels(rng):
     R.ref_process_raw(*dem_args(x, y, z), p(tx), p(rx), n, p(P), 1.0, *sim, S, p(aux), p(raw), p(root))
     res.update(sim_args=sim, sim_S=np.array([S]), sim_aux=aux.reshape(S, n, 2), sim_raw=raw.reshape(S, n, 2),
                sim_root=root.reshape(S, n, 2))
+    # reflection points (calculateReflectionPoint, and as calculaSlantRangeFermat
+    # stores them), from a separate stream so the vectors above stay as they were
+    rr = np.random.default_rng(575)
+    for name, (x, y, z) in dems.items():
+        x, y, z = map(np.ascontiguousarray, (x, y, z))
+        m = 300
+        pr = np.stack([rr.uniform(x.min() - 60, x.max() + 60, m), rr.uniform(y.min() - 60, y.max() + 60, m),
+                       rr.uniform(5, 20, m), rr.uniform(x.min() - 60, x.max() + 60, m),
+                       rr.uniform(y.min() - 60, y.max() + 60, m), rr.uniform(5, 20, m)], 1)
+        pr[:40, 3:5] = pr[:40, 0:2] * -1.0 + 2.0 * np.stack([x[rr.integers(0, len(x), 40)],
+                                                            y[rr.integers(0, len(y), 40)]], 1)   # midpoint on a node
+        pr = np.ascontiguousarray(q6(pr))
+        ro = np.empty((m, 3))
+        R.ref_reflection(*dem_args(x, y, z), p(pr), m, p(ro))
+        res[f"refl_{name}_pairs"], res[f"refl_{name}_out"] = pr, ro
+    e = np.zeros(0)
+    pr = np.ascontiguousarray(np.array([[3.0, -2.0, 12.0, 20.0, 0.5, 11.0]]))
+    ro = np.empty((1, 3))
+    R.ref_reflection(p(e), 0, p(e), 0, p(np.zeros(1)), 0, 0, p(pr), 1, p(ro))
+    res["refl_empty_pairs"], res["refl_empty_out"] = pr, ro
+    x, y, z = dems["bumps"]
+    rs = np.empty((n, 3))
+    R.ref_slant_reflection(*dem_args(x, y, z), p(tx), p(rx), n, p(P), 1.0, 2.449489742783178, p(rs))
+    res["sr_refl"] = rs
     np.savez_compressed(OUT / "ref_kernels.npz", **res)
     print("wrote ref_kernels.npz")
 

@@ -93,6 +93,7 @@ def test_facade_slant_range_bit_exact(tmp_path, golden):
     vals = np.array(rows[:-1])
     assert np.array_equal(vals[:, 0:3], g["sr_ida"]) and np.array_equal(vals[:, 3:6], g["sr_volta"])
     assert np.array_equal(vals[:, 6:10], g["sr_r4"].T)
+    assert np.array_equal(vals[:, 10:13], g["sr_refl"])         # ReflectionPoints, raw_data_processing.cpp:188-193
 
 
 def test_facade_demo_rejects_malformed_input(tmp_path):

@@ -149,9 +149,10 @@ def test_range_compress_fp32(bsar, ctx, oracle, S):
     assert np.abs(gpu - want).max() <= 2e-5 * np.abs(want).max()
 
 
-@pytest.mark.parametrize("n", [7, 12])
+@pytest.mark.parametrize("n", [1, 7, 12])
 def test_range_compress_non_power_of_two(ctx, golden, n):
-    """The reference pads to the next power of two and keeps the first n."""
+    """The reference pads to the next power of two and keeps the first n
+    (length 1: its transforms are the identity)."""
     a = golden[f"fft_{n}_in"].view(np.complex128)
     b = golden[f"corr_{n}_ref"].view(np.complex128)
     gpu = ctx.range_compress(a[None, :], b, conj=True)[0]
@@ -271,3 +272,25 @@ def test_scene_forward_model_vs_oracle(bsar, ctx, oracle, scene_mod):
     assert np.abs(root - want).max() <= 1e-12 * np.abs(want).max()
     raw = scene_mod.simulate(ctx, sc, compress=False)
     assert np.abs(raw - raw_o.T).max() <= 1e-12 * np.abs(raw_o).max()
+
+
+def test_context_dem_cache_follows_the_dem(bsar, ctx, oracle, golden):
+    """The context keeps the last DEM on the device and reuses it only for
+    identical arrays: alternating DEMs of the same sizes (and one changed in
+    a single height) give each DEM's own bit-exact refraction points and
+    heights."""
+    g = golden
+    a = tuple(np.ascontiguousarray(g[f"dem_bumps_{k}"]) for k in "xyz")
+    b = (a[0], a[1], np.ascontiguousarray(a[2] + 0.75))
+    c = (a[0], a[1], a[2].copy())
+    c[2][3, 4] += 1e-9
+    legs = g["refr_bumps_a_legs"]
+    n1, n2 = g["refr_bumps_a_n"]
+    xy = g["dem_bumps_xy"]
+    for dem in (a, b, a, c, a, b):
+        assert bits_equal(ctx.refraction_points(bsar.Dem(*dem), legs, n1, n2), oracle.refraction_points(dem, legs, n1, n2))
+        assert bits_equal(ctx.dem_interp(bsar.Dem(*dem), xy), oracle.dem_interp(dem, xy))
+    e = np.zeros(0)
+    empty = (e, e, np.zeros((0, 0)))
+    assert bits_equal(ctx.refraction_points(bsar.Dem(*empty), g["refr_empty_legs"], 1.0, 2.0), g["refr_empty_out"])
+    assert bits_equal(ctx.refraction_points(bsar.Dem(*a), legs, n1, n2), g["refr_bumps_a_out"])
