@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,16 @@ namespace {
 struct File {
     FILE* f = nullptr;
     ~File() { if (f) std::fclose(f); }
+    // flushes and closes a file that was written: BSAR_ERR_IO when the last
+    // buffered block cannot be written (disk full) or an earlier write failed
+    int close_written()
+    {
+        if (!f) return BSAR_ERR_IO;
+        const bool bad = std::fflush(f) != 0 || std::ferror(f) != 0;
+        const bool closed = std::fclose(f) == 0;
+        f = nullptr;
+        return (bad || !closed) ? BSAR_ERR_IO : BSAR_OK;
+    }
 };
 
 int write_header(FILE* f, uint32_t ndims, const uint64_t* dims)
@@ -43,15 +54,28 @@ int write_csv_vector(const std::string& path, const double* v, int n)
         std::fprintf(fh.f, "%.6f", v[i]);
     }
     std::fputc('\n', fh.f);
-    return std::ferror(fh.f) ? BSAR_ERR_IO : BSAR_OK;
+    return fh.close_written();
 }
 
 }  // namespace
 
 extern "C" {
 
+static int read_matrix_impl(const char* path, uint64_t* dims, int max_dims, int* ndims_out, int* is_complex,
+                            double* buf, uint64_t* count);
+
 int bsar_read_matrix(const char* path, uint64_t* dims, int max_dims, int* ndims_out, int* is_complex,
                      double* buf, uint64_t* count)
+{
+    try {
+        return read_matrix_impl(path, dims, max_dims, ndims_out, is_complex, buf, count);
+    } catch (const std::bad_alloc&) {
+        return BSAR_ERR_NOMEM;
+    }
+}
+
+static int read_matrix_impl(const char* path, uint64_t* dims, int max_dims, int* ndims_out, int* is_complex,
+                            double* buf, uint64_t* count)
 {
     if (!path) return BSAR_ERR_INVALID;
     File fh;
@@ -66,8 +90,12 @@ int bsar_read_matrix(const char* path, uint64_t* dims, int max_dims, int* ndims_
     if (nd > 64) return BSAR_ERR_FORMAT;
     std::vector<uint64_t> d(nd);
     if (nd && std::fread(d.data(), 8, nd, fh.f) != nd) return BSAR_ERR_FORMAT;
+    // element count; a header whose dims overflow 2^60 elements is malformed
     uint64_t total = 1;
-    for (uint64_t x : d) total *= x;
+    for (uint64_t x : d) {
+        if (x && total > ((uint64_t)1 << 60) / x) return BSAR_ERR_FORMAT;
+        total *= x;
+    }
     const bool cplx = (cflag == 1);
     if (ndims_out) *ndims_out = (int)nd;
     if (is_complex) *is_complex = cplx ? 1 : 0;
@@ -100,7 +128,7 @@ int bsar_write_matrix2d(const char* path, const double* data, uint64_t rows, uin
     if (write_header(fh.f, 2, dims)) return BSAR_ERR_IO;
     const uint64_t n = rows * cols * 2;
     if (n && std::fwrite(data, 8, n, fh.f) != n) return BSAR_ERR_IO;
-    return BSAR_OK;
+    return fh.close_written();
 }
 
 int bsar_write_image(const char* path, const double* img, int nx, int ny, int nz, const double* gx,
@@ -117,11 +145,17 @@ int bsar_write_image(const char* path, const double* img, int nx, int ny, int nz
         if (write_header(fh.f, 3, dims)) return BSAR_ERR_IO;
         // data_export.cpp:241-256: z outermost, x fastest -- the layout of img.
         if (total && std::fwrite(img, 16, total, fh.f) != total) return BSAR_ERR_IO;
+        if (int st = fh.close_written()) return st;
     }
     // data_export.cpp:292-300: grid axes beside the image when all are given.
     if (gx && gy && gz && nx > 0 && ny > 0 && nz > 0) {
-        std::string base(path);
-        base = base.substr(0, base.find_last_of('.'));
+        std::string base;
+        try {
+            base = path;
+            base = base.substr(0, base.find_last_of('.'));
+        } catch (const std::bad_alloc&) {
+            return BSAR_ERR_NOMEM;
+        }
         int st = write_csv_vector(base + "_grid_x.csv", gx, nx);
         if (!st) st = write_csv_vector(base + "_grid_y.csv", gy, ny);
         if (!st) st = write_csv_vector(base + "_grid_z.csv", gz, nz);

@@ -37,6 +37,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -301,7 +303,20 @@ double meridian_arc(double a, double f, double lat)
     return k.A * X;
 }
 
+static int geotiff_load_impl(const char* path, bsar_dem_grid* out);
+
 int geotiff_load(const char* path, bsar_dem_grid* out)
+{
+    try {
+        return geotiff_load_impl(path, out);
+    } catch (const std::bad_alloc&) {
+        return file_fail(BSAR_ERR_NOMEM, std::string("GeoTIFF ") + path + ": host allocation failed");
+    } catch (const std::exception& e) {
+        return file_fail(BSAR_ERR_FORMAT, std::string("GeoTIFF ") + path + ": " + e.what());
+    }
+}
+
+static int geotiff_load_impl(const char* path, bsar_dem_grid* out)
 {
     const std::string where = std::string("GeoTIFF ") + path + ": ";
     FILE* fp = std::fopen(path, "rb");
@@ -351,7 +366,13 @@ int geotiff_load(const char* path, bsar_dem_grid* out)
         return values(tag, v) && !v.empty() ? v[0] : dflt;
     };
 
-    const int64_t W = (int64_t)scalar(256, 0), H = (int64_t)scalar(257, 0);
+    // integral tag values in [lo, hi] as int64, else -1 (no out-of-range
+    // double -> integer conversion on crafted tags)
+    auto count_in = [](double v, double lo, double hi) -> int64_t {
+        return (v >= lo && v <= hi && v == std::floor(v)) ? (int64_t)v : -1;
+    };
+    const int64_t W = count_in(scalar(256, 0), 1, (double)(1 << 28));
+    const int64_t H = count_in(scalar(257, 0), 1, (double)(1 << 28));
     const int bps = (int)scalar(258, 1);
     const int comp = (int)scalar(259, 1);
     const int spp = (int)scalar(277, 1);
@@ -376,15 +397,19 @@ int geotiff_load(const char* path, bsar_dem_grid* out)
     int64_t cw, ch;
     const bool tiled = find(322) != nullptr;
     if (tiled) {
-        cw = (int64_t)scalar(322, 0);
-        ch = (int64_t)scalar(323, 0);
+        cw = count_in(scalar(322, 0), 1, (double)(1 << 20));
+        ch = count_in(scalar(323, 0), 1, (double)(1 << 20));
         if (!values(324, offs) || !values(325, cnts)) return file_fail(BSAR_ERR_FORMAT, where + "tile offsets");
     } else {
         cw = W;
-        ch = std::min<int64_t>((int64_t)scalar(278, (double)H), H);
+        const double rps = scalar(278, (double)H);      // RowsPerStrip, often 2^32 - 1 for "one strip"
+        ch = rps >= (double)H ? H : count_in(rps, 1, (double)H);
         if (!values(273, offs) || !values(279, cnts)) return file_fail(BSAR_ERR_FORMAT, where + "strip offsets");
     }
-    if (cw <= 0 || ch <= 0) return file_fail(BSAR_ERR_FORMAT, where + "bad tile/strip size");
+    // a chunk holds at most 2^28 samples (the image bound), so its bytes fit
+    // in 2^31 and every row offset below stays inside `raw`
+    if (cw <= 0 || ch <= 0 || cw * ch > ((int64_t)1 << 28))
+        return file_fail(BSAR_ERR_FORMAT, where + "bad tile/strip size");
     const int64_t across = (W + cw - 1) / cw, down = (H + ch - 1) / ch;
     if ((int64_t)offs.size() < across * down || cnts.size() < offs.size())
         return file_fail(BSAR_ERR_FORMAT, where + "missing tiles/strips");
@@ -394,10 +419,12 @@ int geotiff_load(const char* path, bsar_dem_grid* out)
     for (int64_t cy = 0; cy < down; ++cy)
         for (int64_t cx = 0; cx < across; ++cx) {
             const size_t ci = (size_t)(cy * across + cx);
+            if (!(offs[ci] >= 0 && offs[ci] < 9e15 && cnts[ci] >= 0 && cnts[ci] < 9e15))
+                return file_fail(BSAR_ERR_FORMAT, where + "chunk outside the file");
             const uint64_t off = (uint64_t)offs[ci], cnt = (uint64_t)cnts[ci];
             if (!t.has(off, cnt)) return file_fail(BSAR_ERR_FORMAT, where + "chunk outside the file");
             const int64_t rows = tiled ? ch : std::min(ch, H - cy * ch);
-            const size_t want = (size_t)(cw * rows * bytes);
+            const size_t want = (size_t)((uint64_t)cw * (uint64_t)rows * (uint64_t)bytes);
             raw.assign(want, 0);
             int64_t got;
             switch (comp) {

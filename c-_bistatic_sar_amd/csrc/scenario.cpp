@@ -16,6 +16,8 @@
 #include <cstring>
 #include <fstream>
 #include <sstream>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -188,7 +190,20 @@ BSAR_API int bsar_grid_axis(double center, double step, double half, double* out
     return BSAR_OK;
 }
 
+static int dem_load_impl(const char* path, int mode, bsar_dem_grid* out);
+
 BSAR_API int bsar_dem_load(const char* path, int mode, bsar_dem_grid* out)
+{
+    try {
+        return dem_load_impl(path, mode, out);
+    } catch (const std::bad_alloc&) {
+        return file_fail(BSAR_ERR_NOMEM, "DEM file: host allocation failed");
+    } catch (const std::exception& e) {
+        return file_fail(BSAR_ERR_FORMAT, std::string("DEM file: ") + e.what());
+    }
+}
+
+static int dem_load_impl(const char* path, int mode, bsar_dem_grid* out)
 {
     if (!path || !out) return file_fail(BSAR_ERR_INVALID, "path or out is NULL");
     std::memset(out, 0, sizeof(*out));

@@ -440,6 +440,63 @@ def test_geotiff_rejects_bad_files(bsar, tmp_path):
         bsar.load_dem(f, bsar.DEM_FILE_GEOTIFF)
 
 
+def _patch_tag(path, tag, value):
+    """Rewrite the inline LONG value of `tag` in a little-endian TIFF's first IFD."""
+    b = bytearray(Path(path).read_bytes())
+    ifd = struct.unpack_from("<I", b, 4)[0]
+    n = struct.unpack_from("<H", b, ifd)[0]
+    for k in range(n):
+        o = ifd + 2 + 12 * k
+        t, typ, cnt = struct.unpack_from("<HHI", b, o)
+        if t == tag:
+            assert typ == 4 and cnt == 1
+            struct.pack_into("<I", b, o + 8, value)
+            Path(path).write_bytes(bytes(b))
+            return
+    raise KeyError(tag)
+
+
+@pytest.mark.parametrize("tag,value", [(322, 2 ** 31), (323, 2 ** 31), (322, 2 ** 20 + 1), (322, 0), (256, 2 ** 32 - 1),
+                                       (257, 0), (278, 0)])
+def test_geotiff_rejects_crafted_sizes(bsar, tmp_path, tag, value):
+    """Tile / strip / image sizes from the file are bounded before any
+    allocation or row offset uses them (a chunk holds at most 2^28 samples):
+    crafted tags give a format error, not an overflow, an out-of-bounds read
+    or an uncaught allocation failure."""
+    a = np.arange(16, dtype="<i2").reshape(4, 4)
+    f = tmp_path / "c.tif"
+    _write_tiff(f, a, tile=(16, 16) if tag in (322, 323) else None)
+    _patch_tag(f, tag, value)
+    with pytest.raises(bsar.BsarError) as e:
+        bsar.load_dem(f, bsar.DEM_FILE_GEOTIFF)
+    assert e.value.status == -5
+
+
+def test_geotiff_large_legal_tile_stays_in_bounds(bsar, tmp_path):
+    """A tile of 2^20 x 1 samples over a 4 x 4 raster: the chunk buffer is sized
+    for the tile, the file's short chunk zero-filled, every row read inside it."""
+    a = np.arange(16, dtype="<i2").reshape(4, 4)
+    f = tmp_path / "t.tif"
+    _write_tiff(f, a, tile=(16, 16))
+    _patch_tag(f, 322, 2 ** 20)
+    _patch_tag(f, 323, 1)
+    try:
+        bsar.load_dem(f, bsar.DEM_FILE_GEOTIFF)
+    except bsar.BsarError as e:
+        assert e.status == -5
+
+
+def test_write_reports_close_failure(bsar, tmp_path):
+    """A file whose last buffered block cannot be written (here /dev/full)
+    is reported as an I/O error, not success."""
+    if not Path("/dev/full").exists():
+        pytest.skip("no /dev/full")
+    img = np.zeros((1, 1, 2), np.complex128)
+    with pytest.raises(bsar.BsarError) as e:
+        bsar.write_image("/dev/full", img)
+    assert e.value.status == -4
+
+
 # ---- projection ----------------------------------------------------------------
 
 def test_utm_against_snyder_series(bsar, oracle):
