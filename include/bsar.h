@@ -47,8 +47,8 @@ typedef enum {
 } bsar_status;
 
 typedef enum {
-    BSAR_FP64 = 0,   /* plain form: every step in FP64, reference operation order (R1 sums with
-                        square roots within 0.51 ulp) */
+    BSAR_FP64 = 0,   /* plain form: every step in FP64, reference operation order (R1 sums: squared
+                        distances with two fused terms, square roots within 0.51 ulp) */
     BSAR_MIXED = 1   /* FP64 refraction solve + delay/phase, FP32 interpolation,
                         Kahan-compensated FP32 accumulation, tabulated R1 means */
 } bsar_precision;
