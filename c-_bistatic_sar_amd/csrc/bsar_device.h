@@ -94,6 +94,37 @@ __device__ __forceinline__ void axis_lookup(const double* A, int n, double a0, d
     w = std_max(0.0, std_min(1.0, ww));
 }
 
+// The interval of axis_lookup for an approximate position x on a strictly
+// increasing axis, with an FP32 weight for the screening estimates only:
+// (x - A[i0]) * rcp(A[i0+1] - A[i0]) from FP32 operands (four roundings and
+// v_rcp_f32's 1 ulp), clamped to [0, 1].  Returns false -- the caller then
+// looks up the exact position -- when x lies within eps of the interval's
+// ends: the exact position, within eps of x, could then fall in the
+// neighbouring interval or outside the axis (where the lookup's
+// out-of-range rule jumps to row 0), and with non-finite heights the two
+// intervals need not agree on whether the estimate is NaN.
+__device__ __forceinline__ bool axis_cell_inside(const double* A, int n, double a0, double an, double ih, double x,
+                                                 double eps, int& i0, float& wf)
+{
+    if (!(x - a0 > eps && an - x > eps)) return false;   // a0 = A[0], an = A[n-1]
+    double gd = (x - a0) * ih + 1.0;
+    int g = !(gd > 0.0) ? 0 : (gd >= (double)n ? n : (int)gd);
+    const int gl = g > 0 ? g - 1 : 0, gh = g < n ? g : n - 1;
+    double lo = A[gl], hi = A[gh];
+    if (!(g > 0 && g < n && lo < x && x <= hi)) {
+        while (g > 0 && A[g - 1] >= x) --g;
+        while (g < n && A[g] < x) ++g;
+        g = max(min(g, n - 1), 1);   // A[0] < x <= A[n-1]: g - 1 is the interval
+        lo = A[g - 1];
+        hi = A[g];
+    }
+    if (!(x - lo > eps && hi - x > eps)) return false;
+    i0 = g - 1;
+    const float w = (float)(x - lo) * __builtin_amdgcn_rcpf((float)(hi - lo));
+    wf = __builtin_fminf(__builtin_fmaxf(w, 0.0f), 1.0f);
+    return true;
+}
+
 // interpolateDEMElevation, raw_data_processing.cpp:615-680.
 __device__ __forceinline__ double dem_interp(const DemDev& d, double x, double y)
 {
@@ -141,7 +172,11 @@ __device__ __forceinline__ double dem_interp(const DemDev& d, double x, double y
 // v_sqrt_f32 (1 ulp) give
 //   |est - exact| <= 13 u est + 3 (n1 + n2) 31 u (|Z|max + |z1| + |z2|),
 // i.e. below kRho32 * est + D32 with kRho32 = 4e-6 (5x) and
-// D32 = (n1 + n2) 1e-5 (|Z|max + |z1| + |z2| + 1) (1.6x).  A candidate is
+// D32 = (n1 + n2) 1e-5 (|Z|max + |z1| + |z2| + 1) (1.6x).  The column terms
+// come from a position within epsy (~1e-11 m) of the candidate's and an FP32
+// column weight (axis_cell_inside: 5 u relative, i.e. at most 10 u |Z|max more
+// on a height, (n1 + n2) 10 u |Z|max on the estimate), which the 1.6x leaves
+// room for (75 u of the 1e-5).  A candidate is
 // dropped only when est > M (1 + 2 kRho32) + 2 D32, which puts its exact
 // value above M; M itself only takes exact values or est upper bounds
 // (est + D32)(1 + 3 kRho32).  Any candidate dropped by a bound or an estimate
@@ -430,14 +465,26 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
             continue;
         }
         colmask |= 1u << t;
-        const double py = minY + j * spanY / (G - 1);
+        // Terms of the FP32 estimates only (the exact evaluations look the
+        // column up again at its exact position): pya is within epsy of it,
+        // and away from the axis ends (where the out-of-range rule of the
+        // lookup jumps to row 0) its interval and an FP32 weight serve; near
+        // them, or on an unsorted axis, the exact position and lookup.
         int j0 = 0, dj = 0;
-        double w = 0.0;
-        if (!empty) axis_lookup(d.Y, d.ny, d.y0, d.ihy, ysorted, py, j0, dj, w);
+        float wf = 0.0f;
+        if (!empty) {
+            if (ysorted && axis_cell_inside(d.Y, d.ny, d.y0, d.yn, d.ihy, pya, epsy, j0, wf)) {
+                dj = 1;
+            } else {
+                double w = 0.0;
+                axis_lookup(d.Y, d.ny, d.y0, d.ihy, ysorted, minY + j * spanY / (G - 1), j0, dj, w);
+                wf = (float)w;
+            }
+        }
         o0[t] = j0 * d.zs;
         o1[t] = (j0 + dj) * d.zs;
-        wyf[t] = (float)w;
-        const double dy1 = py - L.y1, dy2 = L.y2 - py;
+        wyf[t] = wf;
+        const double dy1 = pya - L.y1, dy2 = L.y2 - pya;
         sy1[t] = (float)(dy1 * dy1);
         sy2[t] = (float)(dy2 * dy2);
     }

@@ -294,3 +294,25 @@ def test_context_dem_cache_follows_the_dem(bsar, ctx, oracle, golden):
     empty = (e, e, np.zeros((0, 0)))
     assert bits_equal(ctx.refraction_points(bsar.Dem(*empty), g["refr_empty_legs"], 1.0, 2.0), g["refr_empty_out"])
     assert bits_equal(ctx.refraction_points(bsar.Dem(*a), legs, n1, n2), g["refr_bumps_a_out"])
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_refraction_with_non_finite_heights_vs_oracle(bsar, ctx, oracle, scene_mod, bad):
+    """DEM nodes that are NaN or infinite next to the refraction region: the
+    reference's strict '<' never selects a NaN path and the screening must not
+    drop the candidate that wins; refraction points and images bit-exact /
+    within 1e-9 of the oracle."""
+    rng = np.random.default_rng(11)
+    x, y, z = scene_mod.dem_bumps(rng)
+    z = z.copy()
+    ix, iy = np.searchsorted(x, 20.0), np.searchsorted(y, 0.0)
+    for dj, di in ((0, 0), (0, -1), (1, 1), (-1, 2)):
+        z[iy + dj, ix + di] = bad
+    dem = (x, y, z)
+    M = 1500
+    top = np.stack([rng.uniform(-15, 15, M), rng.uniform(-15, 15, M), rng.uniform(10, 14, M)], 1)
+    bot = np.stack([rng.uniform(-10, 50, M), rng.uniform(-30, 30, M), rng.uniform(-6, -4, M)], 1)
+    legs = np.concatenate([top, bot], 1)
+    legs[M // 2:] = np.concatenate([bot, top], 1)[M // 2:]
+    n2 = 2.449489742783178
+    assert bits_equal(ctx.refraction_points(bsar.Dem(*dem), legs, 1.0, n2), oracle.refraction_points(dem, legs, 1.0, n2))
