@@ -111,7 +111,8 @@ def _num(row: dict, key: str):
         return None
 
 
-def roofline(rep: dict, n: int, form: str) -> dict:
+def roofline(rep: dict, n: int, form: str, flops_prof: str = PROFILE_FLOPS, traffic_prof: str = PROFILE_TRAFFIC,
+             issue_prof: str = PROFILE_ISSUE) -> dict:
     """Roofline of the dominant back-projection kernel of one image.
 
     achieved = executed FP64 + FP32 flop of one launch / the launch's average
@@ -130,9 +131,9 @@ def roofline(rep: dict, n: int, form: str) -> dict:
     t_launch = rep["seconds"] / launches
     vp_launch = rep["voxel_pulses"] / launches
     kernel = "k_bp_mixed" if form == "mixed" else "k_bp_fp64"
-    flops, fcommit = profile_table(PROFILE_FLOPS)
-    traffic, tcommit = profile_table(PROFILE_TRAFFIC)
-    issue, icommit = profile_table(PROFILE_ISSUE)
+    flops, fcommit = profile_table(flops_prof)
+    traffic, tcommit = profile_table(traffic_prof)
+    issue, icommit = profile_table(issue_prof)
     row = flops.get(kernel, {})
     e64, e32 = _num(row, "fp64 flop/vp"), _num(row, "fp32 flop/vp")
     out = {"bound": "valu", "achieved": None, "peak": P64 / 1e12, "unit": "TFLOP/s", "frac": None,
@@ -142,16 +143,16 @@ def roofline(rep: dict, n: int, form: str) -> dict:
         ach = (e64 + e32) * vp_launch / t_launch
         out.update(achieved=round(ach / 1e12, 3), frac=round(ach / P64, 4),
                    executed_flop_per_voxel_pulse={"fp64": e64, "fp32": e32,
-                                                  "source": f"profiles/{PROFILE_FLOPS}",
+                                                  "source": f"profiles/{flops_prof}",
                                                   "commit": fcommit})
     tb = _num(traffic.get(kernel, {}), "bytes per launch")
     if tb is not None:
         out["traffic"] = int(tb)
-        out["traffic_source"] = {"source": f"profiles/{PROFILE_TRAFFIC}", "commit": tcommit,
+        out["traffic_source"] = {"source": f"profiles/{traffic_prof}", "commit": tcommit,
                                  "algorithmic_bytes_per_launch": _num(traffic[kernel], "algorithmic bytes")}
     iu = _num(issue.get(kernel, {}), "issue util")
     if iu is not None:
-        out["valu_issue_util"] = {"value": iu, "source": f"profiles/{PROFILE_ISSUE}", "commit": icommit}
+        out["valu_issue_util"] = {"value": iu, "source": f"profiles/{issue_prof}", "commit": icommit}
     eff = (f64 + f32) * vp_launch / t_launch
     out["effective_tflops"] = round(eff / 1e12, 3)
     out["reference_flop_per_voxel_pulse"] = {"fp64": f64, "fp32": f32}
@@ -462,7 +463,11 @@ def run_rough(args, pkg, scene) -> dict:
     dtau = max(float(np.abs(delay(la, ga, sc.n1, sc.n2) - delay(la, ra, sc.n1, sc.n2)).max()),
                float(np.abs(delay(lb, gb, sc.n2, sc.n1) - delay(lb, rb, sc.n2, sc.n1)).max()))
     log(f"argmin check: {same}/{n_check} identical, max |dtau| = {dtau:.3e} s ({kind} CPU, {cpu_s:.1f} s)")
-    ach = value * FLOP_PER_SOLVE
+    ach = value * FLOP_PER_SOLVE                 # the reference algorithm's flop
+    rows, rcommit = profile_table("valu_flops_slant_rough.md")
+    e64, e32 = _num(rows.get("k_slant_range", {}), "fp64 flop/solve"), _num(rows.get("k_slant_range", {}),
+                                                                             "fp32 flop/solve")
+    executed = None if e64 is None or e32 is None else value * (e64 + e32)
     return {"metric": "refraction solves/s (Fermat search over the DEM, FP64), both legs of every voxel-pulse",
             "value": round(value, 1), "unit": "solve/s", "n_gpus": 1, "steps": 1, "warmup": 0,
             "ms_per_step": round(dev_s * 1e3, 1), "higher_is_better": True, "scaling": "weak",
@@ -472,9 +477,14 @@ def run_rough(args, pkg, scene) -> dict:
                                    f"{sc.n2 ** 2:.3f}", "voxels": V, "pulses": N, "solves": solves,
                        "chunk_voxels": chunk},
             "wall_s": round(wall, 2),
-            "roofline": {"bound": "valu", "achieved": round(ach / 1e12, 3), "peak": P64 / 1e12,
-                         "unit": "TFLOP/s", "frac": round(ach / P64, 4), "traffic": None,
-                         "kernel": "k_slant_range", "flop_per_solve": FLOP_PER_SOLVE},
+            "roofline": {"bound": "valu",
+                         "achieved": None if executed is None else round(executed / 1e12, 3), "peak": P64 / 1e12,
+                         "unit": "TFLOP/s", "frac": None if executed is None else round(executed / P64, 4),
+                         "traffic": None, "kernel": "k_slant_range",
+                         "executed_flop_per_solve": None if executed is None else {
+                             "fp64": e64, "fp32": e32, "source": "profiles/valu_flops_slant_rough.md",
+                             "commit": rcommit},
+                         "effective_tflops": round(ach / 1e12, 3), "reference_flop_per_solve": FLOP_PER_SOLVE},
             "argmin_check": {"solves": n_check, "identical": same, "max_abs_dtau_s": dtau, "cpu": kind},
             "cpu_baseline": {"value": round(n_check / cpu_s, 1), "unit": "solve/s", "cores": threads, "kind": kind,
                              "sample": f"{n_check} solves (the argmin-check sample: {len(legs_a)} voxels x "
@@ -554,7 +564,7 @@ def run_wide(args, pkg, scene) -> dict:
                                   f"x{sc.upsample}, 64 scatterers, B 500 MHz", "voxels": V, "pulses": N,
                       "samples": S},
            "wall_s": round(wall_m, 2),
-           "roofline": roofline(rep, N, "mixed"),
+           "roofline": roofline(rep, N, "mixed", "valu_flops_bp_wide.md", "none", "none"),
            "range_compression": {"fp32_ms": round(rc32["seconds"] * 1e3, 3), "fp64_ms": round(rc64["seconds"] * 1e3, 3),
                                  "fp32_gb_s": round(gbs, 1), "roofline_frac": round(gbs * 1e9 / 8e12, 4),
                                  "algorithmic_bytes": bytes32, "bound": "hbm",
