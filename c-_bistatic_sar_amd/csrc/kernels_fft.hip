@@ -9,10 +9,12 @@
 // workgroup: the column is staged in LDS, transformed in place (radix-2 DIT
 // forward from bit-reversed load, DIF inverse to bit-reversed store, three
 // stages per LDS round trip, twiddles from a precomputed table), so HBM sees
-// one read and one write per sample.  Columns of twice the LDS budget are
-// split by one global radix-2 stage into two half-length columns filtered in
-// LDS and merged by one global radix-2 stage (three passes over the data);
-// longer ones run as log2(S) global Stockham passes per direction.
+// one read and one write per sample.  16384-sample complex128 columns (twice
+// the LDS budget) go through LDS half a column at a time; other columns of
+// twice the budget are split by one global radix-2 stage into two
+// half-length columns filtered in LDS and merged by one global radix-2 stage
+// (three passes over the data); longer ones run as log2(S) global Stockham
+// passes per direction.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -161,27 +163,30 @@ __device__ __forceinline__ void lds_turn(typename C2<T>::type* x, int q, int log
 }
 
 // The stages below distance 2^top in LDS (top = log2 of the register pass's
-// smallest distance, >= 1): forward DIF in radix-16 passes from the top, the
-// innermost Kb = top mod 4 (or 4) stages fused with the product and the
-// inverse's innermost stages, then the inverse DIT in radix-16 passes
-// outwards.
-template <typename T, int Kb>
-__device__ __forceinline__ void lds_filter(typename C2<T>::type* x, int top, int logS,
+// smallest distance, >= 1): forward DIF in radix-2^R passes from the top, the
+// innermost Kb (top - Kb a multiple of R, 1 <= Kb <= R) stages fused with the
+// product and the inverse's innermost stages, then the inverse DIT in
+// radix-2^R passes outwards.  x holds 2^logN elements: whole blocks of 2^top,
+// which the LDS stages never mix, so any run of blocks of the column; logS is
+// the column's length (it sets the twiddles) and Hbr the spectrum from x's
+// first element on.
+template <typename T, int Kb, int R = 4>
+__device__ __forceinline__ void lds_filter(typename C2<T>::type* x, int top, int logN, int logS,
                                            const typename C2<T>::type* __restrict__ Hbr,
                                            const typename C2<T>::type* __restrict__ tw)
 {
-    const int S = 1 << logS;
-    for (int s = top; s > Kb; s -= 4) {
+    const int N = 1 << logN;
+    for (int s = top; s > Kb; s -= R) {
 #pragma unroll 1
-        for (int q = threadIdx.x; q < (S >> 4); q += blockDim.x) lds_group<T, 4, true, false>(x, s - 4, q, logS, tw);
+        for (int q = threadIdx.x; q < (N >> R); q += blockDim.x) lds_group<T, R, true, false>(x, s - R, q, logS, tw);
         __syncthreads();
     }
 #pragma unroll 1
-    for (int q = threadIdx.x; q < (S >> Kb); q += blockDim.x) lds_turn<T, Kb>(x, q, logS, Hbr, tw);
+    for (int q = threadIdx.x; q < (N >> Kb); q += blockDim.x) lds_turn<T, Kb>(x, q, logS, Hbr, tw);
     __syncthreads();
-    for (int s = Kb; s < top; s += 4) {
+    for (int s = Kb; s < top; s += R) {
 #pragma unroll 1
-        for (int q = threadIdx.x; q < (S >> 4); q += blockDim.x) lds_group<T, 4, false, true>(x, s, q, logS, tw);
+        for (int q = threadIdx.x; q < (N >> R); q += blockDim.x) lds_group<T, R, false, true>(x, s, q, logS, tw);
         __syncthreads();
     }
 }
@@ -192,22 +197,33 @@ template <typename T>
 constexpr int fft_max_threads() { return (int)(kFftLdsMax / sizeof(typename C2<T>::type) / 16); }
 
 // Fused filter: out[c] = IFFT(FFT(in[c]) .* H) for one column c per block of
-// S / 2^KP threads (Kb = the innermost LDS stages, (logS - KP) mod 4 or 4).  Thread t owns samples t + i S/2^KP (i < 2^KP): it loads
+// S / 2^KP threads.  Thread t owns samples t + i S/2^KP (i < 2^KP): it loads
 // them coalesced, runs the KP largest-distance DIF stages in registers and
 // stores them to LDS; the remaining stages, the product with the spectrum in
-// bit-reversed order (Hbr = H permuted) and the inverse's matching stages
-// run in LDS; the thread then reads its samples back, runs the KP
-// largest-distance DIT stages of the inverse in registers and stores the
-// samples / S coalesced.  tw[k] = exp(-2 pi i k / S), k < S / 2.
-template <typename T, int KP, int Kb>
-__global__ __launch_bounds__(fft_max_threads<T>()) void k_fft_filter_lds(const typename C2<T>::type* __restrict__ in,
-                                                         typename C2<T>::type* __restrict__ out,
-                                                         const typename C2<T>::type* __restrict__ Hbr,
-                                                         const typename C2<T>::type* __restrict__ tw, int logS,
-                                                         int halves)
+// bit-reversed order (Hbr = H permuted) and the inverse's matching stages run
+// in LDS (lds_filter: radix-2^R passes, Kb innermost stages fused); the thread
+// then reads its samples back, runs the KP largest-distance DIT stages of the
+// inverse in registers and stores the samples / S coalesced.
+// tw[k] = exp(-2 pi i k / S), k < S / 2.
+//
+// The register stages leave 2^KP independent blocks of S / 2^KP samples, so
+// with NH = 2 the column goes through LDS one half at a time (a thread's
+// registers i < P/2, then the rest): half the LDS per workgroup.  The lean
+// variants (R = 3, MINW waves per SIMD, NT threads at most) use that for two
+// workgroups per CU, so one column's loads and stores overlap another's LDS
+// passes, or for columns twice the whole-column LDS budget.
+template <typename T, int KP, int Kb, int NH = 1, int R = 4, int MINW = 1, int NT = fft_max_threads<T>()>
+__global__ __launch_bounds__(NT, MINW) void k_fft_filter_lds(const typename C2<T>::type* __restrict__ in,
+                                                            typename C2<T>::type* __restrict__ out,
+                                                            const typename C2<T>::type* __restrict__ Hbr,
+                                                            const typename C2<T>::type* __restrict__ tw, int logS,
+                                                            int halves)
 {
     using V = typename C2<T>::type;
     constexpr int P = 1 << KP;
+    constexpr int PH = P / NH;              // registers per LDS phase
+    constexpr int LNH = NH == 2 ? 1 : 0;
+    static_assert(NH == 1 || NH == 2, "one or two LDS phases");
     extern __shared__ unsigned char smem[];
     V* x = reinterpret_cast<V*>(smem);
     const int S = 1 << logS;
@@ -227,11 +243,15 @@ __global__ __launch_bounds__(fft_max_threads<T>()) void k_fft_filter_lds(const t
         for (int i = 0; i < P; ++i) v[i] = cmul(v[i], Hc[i]);
     } else {
 #pragma unroll
-        for (int i = 0; i < P; ++i) x[lds_pad(t + i * nt)] = v[i];
-        __syncthreads();
-        lds_filter<T, Kb>(x, top, logS, Hc, tw);
+        for (int h = 0; h < NH; ++h) {
+            if (h) __syncthreads();         // every read of the previous phase is done
 #pragma unroll
-        for (int i = 0; i < P; ++i) v[i] = x[lds_pad(t + i * nt)];
+            for (int i = 0; i < PH; ++i) x[lds_pad(t + i * nt)] = v[h * PH + i];
+            __syncthreads();
+            lds_filter<T, Kb, R>(x, top, logS - LNH, logS, Hc + (int64_t)h * (S >> LNH), tw);
+#pragma unroll
+            for (int i = 0; i < PH; ++i) v[h * PH + i] = x[lds_pad(t + i * nt)];
+        }
     }
     butterflies<T, KP, false, true>(v, t, top, logS, tw);
     const T inv = (T)1 / (T)S;
@@ -424,6 +444,24 @@ static hipError_t launch_lds_filter(typename C2<T>::type* data, const typename C
     return hipGetLastError();
 }
 
+// A lean variant of the filter: NH LDS phases, radix-2^R LDS passes, sized
+// for MINW waves per SIMD, up to NT threads.
+template <typename T, int KP, int Kb, int NH, int R, int MINW, int NT>
+static hipError_t launch_lds_filter_lean(typename C2<T>::type* data, const typename C2<T>::type* Hbr,
+                                         const typename C2<T>::type* tw, int logS, int64_t ncols, hipStream_t st,
+                                         int halves)
+{
+    const int S = 1 << logS;
+    if ((S >> KP) > NT || logS - KP < 1 || (logS - KP - Kb) % R != 0) return hipErrorInvalidValue;
+    const size_t lb = lds_bytes<T>(S / NH);
+    hipError_t e = hipFuncSetAttribute((const void*)k_fft_filter_lds<T, KP, Kb, NH, R, MINW, NT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_fft_filter_lds<T, KP, Kb, NH, R, MINW, NT>), dim3((unsigned)ncols), dim3(S >> KP), lb, st,
+                       data, data, Hbr, tw, logS, halves);
+    return hipGetLastError();
+}
+
 // Twiddle tables tw[k] = exp(-2 pi i k / S), k < S / 2, one per (device, S,
 // precision), built on first use and kept for the life of the process.
 template <typename T>
@@ -478,7 +516,12 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
             }
         }
     };
-    if ((size_t)S * sizeof(V) <= kFftLdsMax) {
+    // 16384 complex128 samples (256 KiB) go through LDS in two halves with
+    // radix-8 passes in one 1024-thread workgroup per CU: one pass over HBM
+    // instead of the split path's three (the kernel spills 224 B per lane at
+    // its 128 VGPRs, radix-16 passes 472 B).
+    const bool lean = std::is_same<T, double>::value && logS == 14;
+    if (lean || (size_t)S * sizeof(V) <= kFftLdsMax) {
         // S / 2^KP <= 1024 threads: S <= 16384 (complex64) / 8192 (complex128)
         if (!tmp) return hipErrorInvalidValue;
         const V* tw = tw_given;
@@ -486,6 +529,9 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_bitrev_copy<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        if constexpr (std::is_same<T, double>::value) {
+            if (lean) return launch_lds_filter_lean<T, 4, 1, 2, 3, 4, 1024>(data, tmp, tw, logS, ncols, st, 0);
+        }
         return lds(logS, tmp, tw, ncols, 0);
     }
     if ((size_t)(S / 2) * sizeof(V) <= kFftLdsMax) {
