@@ -12,7 +12,8 @@ constexpr size_t kFftLdsMax = 128 * 1024;
 
 // data[c] <- IFFT(FFT(data[c]) .* H) for ncols contiguous columns of length
 // S (power of two).  Columns within the LDS budget run in LDS (16384
-// complex128 samples too, half of the column at a time); other columns of
+// complex128 samples too, half of the column at a time; 16384-sample columns
+// in a persistent workgroup per CU); other columns of
 // twice that as two half-length LDS filters between a global radix-2 split
 // and merge; longer ones as global radix-2 passes.  tmp: scratch of
 // fft_filter_tmp_elems(S, ncols, bytes per complex) complex elements -- S

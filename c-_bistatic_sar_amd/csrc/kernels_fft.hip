@@ -9,15 +9,18 @@
 // workgroup: the column is staged in LDS, transformed in place (radix-2 DIT
 // forward from bit-reversed load, DIF inverse to bit-reversed store, three
 // stages per LDS round trip, twiddles from a precomputed table), so HBM sees
-// one read and one write per sample.  16384-sample complex128 columns (twice
-// the LDS budget) go through LDS half a column at a time; other columns of
-// twice the budget are split by one global radix-2 stage into two
-// half-length columns filtered in LDS and merged by one global radix-2 stage
-// (three passes over the data); longer ones run as log2(S) global Stockham
-// passes per direction.
+// one read and one write per sample.  16384-sample columns run in a
+// persistent filter with the pass structure fixed at compile time
+// (k_fft_filter_pc: complex64 with the next column prefetched during the
+// inverse stages, complex128 -- twice the LDS budget -- half a column at a
+// time); other columns of twice the budget are split by one global radix-2
+// stage into two half-length columns filtered in LDS and merged by one global
+// radix-2 stage (three passes over the data); longer ones run as log2(S)
+// global Stockham passes per direction.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -65,6 +68,74 @@ __device__ __forceinline__ typename C2<T>::type rot16(int r, int d, T sgn)
     }
 }
 
+// a * b with fused products (two roundings per component instead of three).
+__device__ __forceinline__ float2 cmul_f(float2 a, float2 b)
+{
+    return float2{__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x)};
+}
+__device__ __forceinline__ double2 cmul_f(double2 a, double2 b)
+{
+    return double2{__builtin_fma(a.x, b.x, -(a.y * b.y)), __builtin_fma(a.x, b.y, a.y * b.x)};
+}
+
+// exp(sgn i pi k / 16), k < 16.
+template <typename T>
+__device__ __forceinline__ typename C2<T>::type rot_16th(int k, T sgn)
+{
+    using V = typename C2<T>::type;
+    constexpr T a1 = (T)0.98078528040323044913;   // cos(pi/16)
+    constexpr T b1 = (T)0.19509032201612826785;   // sin(pi/16)
+    constexpr T a2 = (T)0.92387953251128675613;   // cos(pi/8)
+    constexpr T b2 = (T)0.38268343236508977173;   // sin(pi/8)
+    constexpr T a3 = (T)0.83146961230254523708;   // cos(3 pi/16)
+    constexpr T b3 = (T)0.55557023301960222474;   // sin(3 pi/16)
+    switch (k) {
+    case 1: return V{a1, sgn * b1};
+    case 2: return V{a2, sgn * b2};
+    case 3: return V{a3, sgn * b3};
+    case 5: return V{b3, sgn * a3};
+    case 6: return V{b2, sgn * a2};
+    case 7: return V{b1, sgn * a1};
+    case 9: return V{-b1, sgn * a1};
+    case 10: return V{-b2, sgn * a2};
+    case 11: return V{-b3, sgn * a3};
+    case 13: return V{-a3, sgn * b3};
+    case 14: return V{-a2, sgn * b2};
+    default: return V{-a1, sgn * b1};   // 15
+    }
+}
+
+// a * exp(sgn i pi r / d) for compile-time r < d <= 16: the quarter turns as
+// swaps and signs (exact), the odd eighths as a sum and a difference scaled
+// by cos(pi/4), the rest as a fused product.
+template <typename T>
+__device__ __forceinline__ typename C2<T>::type crot_f(typename C2<T>::type a, int r, int d, T sgn)
+{
+    using V = typename C2<T>::type;
+    constexpr T c4 = (T)0.70710678118654752440;
+    const int k = r * (16 / d);                   // angle k pi / 16
+    if (k == 0) return a;
+    if (k == 8) return V{-sgn * a.y, sgn * a.x};
+    if (k == 4) return V{c4 * (a.x - sgn * a.y), c4 * (a.y + sgn * a.x)};
+    if (k == 12) return V{-c4 * (a.x + sgn * a.y), c4 * (sgn * a.x - a.y)};
+    return cmul_f(a, rot_16th<T>(k, sgn));
+}
+
+// Twiddle sources: the table itself, or (TwLds) its two-level copy in LDS for
+// S = 2^14, tw[k] = c[k / 64] * f[k mod 64] with c[a] = tw[64 a] (a < 128) at
+// t[0, 128) and f[b] = tw[b] at t[128, 192) -- one more rounding.
+template <typename V>
+__device__ __forceinline__ V tw_get(const V* __restrict__ tw, int64_t k) { return tw[k]; }
+
+template <typename V>
+struct TwLds { const V* t; };
+template <typename V>
+__device__ __forceinline__ V tw_get(TwLds<V> tl, int64_t k)
+{
+    const uint32_t u = (uint32_t)k;
+    return cmul_f(tl.t[u >> 6], tl.t[128 + (u & 63)]);
+}
+
 // K consecutive radix-2 stages on the 2^K elements v[m] = x[e0 + m h] (h =
 // 2^lh, j = e0 mod h).  The stage at distance 2^t h pairs v[m], v[m + 2^t]
 // with twiddle exp(sgn i pi (j + (m mod 2^t) h) / (2^t h)) = W_t *
@@ -75,12 +146,15 @@ __device__ __forceinline__ typename C2<T>::type rot16(int r, int d, T sgn)
 // (distances ascending, twiddle on the second input: bit-reversed in,
 // natural out).
 // kUnit: j = 0 (W = 1), the twiddles are the constant rotations alone.
-template <typename T, int K, bool kDif, bool kInv, bool kUnit = false>
-__device__ __forceinline__ void butterflies(typename C2<T>::type* v, int j, int lh, int logS,
-                                            const typename C2<T>::type* __restrict__ tw)
+// kFast: fused products (cmul_f) and exact quarter turns (crot_f) -- fewer
+// instructions, different last bits; TWS: the twiddle source (tw_get).
+template <typename T, int K, bool kDif, bool kInv, bool kUnit = false, bool kFast = false,
+          typename TWS = const typename C2<T>::type*>
+__device__ __forceinline__ void butterflies(typename C2<T>::type* v, int j, int lh, int logS, TWS tw)
 {
     using V = typename C2<T>::type;
     constexpr int G = 1 << K;
+    static_assert(K <= 4 || (kFast && K <= 5), "constant rotations: radix 16, radix 32 with kFast");
     const T sgn = kInv ? (T)1 : (T)-1;
     V wt[K];
     if (kUnit) {
@@ -90,7 +164,7 @@ __device__ __forceinline__ void butterflies(typename C2<T>::type* v, int j, int 
         // W_t = exp(sgn i pi j / (2^t h)) = tw[j S / (2^(t+1) h)], read from the table
 #pragma unroll
         for (int t = 0; t < K; ++t) {
-            V w = tw[(int64_t)j << (logS - lh - 1 - t)];
+            V w = tw_get(tw, (int64_t)j << (logS - lh - 1 - t));
             if (kInv) w.y = -w.y;
             wt[t] = w;
         }
@@ -103,6 +177,20 @@ __device__ __forceinline__ void butterflies(typename C2<T>::type* v, int j, int 
         for (int m = 0; m < G; ++m) {
             if (m & d) continue;
             const int r = m & (d - 1);
+            if (kFast) {
+                if (kDif) {
+                    const V u = v[m], b = v[m + d];
+                    v[m] = cadd(u, b);
+                    const V df = csub(u, b);
+                    v[m + d] = kUnit ? crot_f<T>(df, r, d, sgn) : cmul_f(df, crot_f<T>(wt[t], r, d, sgn));
+                } else {
+                    const V u = v[m];
+                    const V b = kUnit ? crot_f<T>(v[m + d], r, d, sgn) : cmul_f(v[m + d], crot_f<T>(wt[t], r, d, sgn));
+                    v[m] = cadd(u, b);
+                    v[m + d] = csub(u, b);
+                }
+                continue;
+            }
             const V tw_m = kUnit ? rot16<T>(r, d, sgn) : ((r == 0) ? wt[t] : cmul(wt[t], rot16<T>(r, d, sgn)));
             const bool trivial = kUnit && r == 0;
             if (kDif) {
@@ -205,25 +293,15 @@ constexpr int fft_max_threads() { return (int)(kFftLdsMax / sizeof(typename C2<T
 // then reads its samples back, runs the KP largest-distance DIT stages of the
 // inverse in registers and stores the samples / S coalesced.
 // tw[k] = exp(-2 pi i k / S), k < S / 2.
-//
-// The register stages leave 2^KP independent blocks of S / 2^KP samples, so
-// with NH = 2 the column goes through LDS one half at a time (a thread's
-// registers i < P/2, then the rest): half the LDS per workgroup.  The lean
-// variants (R = 3, MINW waves per SIMD, NT threads at most) use that for two
-// workgroups per CU, so one column's loads and stores overlap another's LDS
-// passes, or for columns twice the whole-column LDS budget.
-template <typename T, int KP, int Kb, int NH = 1, int R = 4, int MINW = 1, int NT = fft_max_threads<T>()>
-__global__ __launch_bounds__(NT, MINW) void k_fft_filter_lds(const typename C2<T>::type* __restrict__ in,
-                                                            typename C2<T>::type* __restrict__ out,
-                                                            const typename C2<T>::type* __restrict__ Hbr,
-                                                            const typename C2<T>::type* __restrict__ tw, int logS,
-                                                            int halves)
+template <typename T, int KP, int Kb>
+__global__ __launch_bounds__(fft_max_threads<T>()) void k_fft_filter_lds(const typename C2<T>::type* __restrict__ in,
+                                                                        typename C2<T>::type* __restrict__ out,
+                                                                        const typename C2<T>::type* __restrict__ Hbr,
+                                                                        const typename C2<T>::type* __restrict__ tw,
+                                                                        int logS, int halves)
 {
     using V = typename C2<T>::type;
     constexpr int P = 1 << KP;
-    constexpr int PH = P / NH;              // registers per LDS phase
-    constexpr int LNH = NH == 2 ? 1 : 0;
-    static_assert(NH == 1 || NH == 2, "one or two LDS phases");
     extern __shared__ unsigned char smem[];
     V* x = reinterpret_cast<V*>(smem);
     const int S = 1 << logS;
@@ -243,20 +321,177 @@ __global__ __launch_bounds__(NT, MINW) void k_fft_filter_lds(const typename C2<T
         for (int i = 0; i < P; ++i) v[i] = cmul(v[i], Hc[i]);
     } else {
 #pragma unroll
-        for (int h = 0; h < NH; ++h) {
-            if (h) __syncthreads();         // every read of the previous phase is done
+        for (int i = 0; i < P; ++i) x[lds_pad(t + i * nt)] = v[i];
+        __syncthreads();
+        lds_filter<T, Kb>(x, top, logS, logS, Hc, tw);
 #pragma unroll
-            for (int i = 0; i < PH; ++i) x[lds_pad(t + i * nt)] = v[h * PH + i];
-            __syncthreads();
-            lds_filter<T, Kb, R>(x, top, logS - LNH, logS, Hc + (int64_t)h * (S >> LNH), tw);
-#pragma unroll
-            for (int i = 0; i < PH; ++i) v[h * PH + i] = x[lds_pad(t + i * nt)];
-        }
+        for (int i = 0; i < P; ++i) v[i] = x[lds_pad(t + i * nt)];
     }
     butterflies<T, KP, false, true>(v, t, top, logS, tw);
     const T inv = (T)1 / (T)S;
 #pragma unroll
     for (int i = 0; i < P; ++i) out[col * S + t + i * nt] = V{v[i].x * inv, v[i].y * inv};
+}
+
+// ---- persistent filter for 16384-sample columns --------------------------
+//
+// The transform of k_fft_filter_lds with the column length, the thread count
+// and every pass fixed at compile time (LDS offsets and twiddle shifts are
+// immediates), fused products and exact quarter turns (butterflies' kFast),
+// and one workgroup per CU looping over columns.  Twiddles come from the
+// two-level table in LDS and LDS exchanges use lds_barrier, so the only
+// global accesses are the samples and the spectrum: with PF the next
+// column's samples load into registers while the inverse stages run.
+
+// Barrier for LDS exchanges: waits for the wave's LDS operations but not its
+// global loads and stores (__syncthreads waits for those as well).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// A work item's elements e0 + m h (h = 2^lh, e0 = a multiple of 2^(lh + K)
+// plus j < h) sit at lds_pad(e0) + m h + (m h) / 32: below h = 32 the item's
+// elements either stay inside one 32-element block or, from e0 mod 32 = j <
+// h, never carry into the next block's pad; from h = 32 on each step m h is
+// whole blocks.
+__device__ __forceinline__ int lds_step(int m, int lh) { return (m << lh) + ((m << lh) >> 5); }
+
+template <typename T, int K, bool kDif, bool kInv, int LH, int LOGS, int LOGN, int NT, typename TWS>
+__device__ __forceinline__ void lds_pass_ct(typename C2<T>::type* x, TWS tw)
+{
+    using V = typename C2<T>::type;
+    constexpr int G = 1 << K, h = 1 << LH, items = (1 << LOGN) >> K;
+    static_assert(items % NT == 0, "whole items per thread");
+#pragma unroll 1
+    for (int q0 = 0; q0 < items; q0 += NT) {
+        const int q = q0 + (int)threadIdx.x;
+        const int j = q & (h - 1);
+        V* xb = x + lds_pad(((q >> LH) << (LH + K)) + j);
+        V v[G];
+#pragma unroll
+        for (int m = 0; m < G; ++m) v[m] = xb[lds_step(m, LH)];
+        butterflies<T, K, kDif, kInv, false, true, TWS>(v, j, LH, LOGS, tw);
+#pragma unroll
+        for (int m = 0; m < G; ++m) xb[lds_step(m, LH)] = v[m];
+    }
+}
+
+// Forward DIF passes from distance 2^S_ down to 2^Kb (radix 2^R).
+template <typename T, int Kb, int R, int S_, int LOGS, int LOGN, int NT, typename TWS>
+__device__ __forceinline__ void lds_fwd_ct(typename C2<T>::type* x, TWS tw)
+{
+    if constexpr (S_ > Kb) {
+        lds_pass_ct<T, R, true, false, S_ - R, LOGS, LOGN, NT, TWS>(x, tw);
+        lds_barrier();
+        lds_fwd_ct<T, Kb, R, S_ - R, LOGS, LOGN, NT, TWS>(x, tw);
+    }
+}
+
+// Inverse DIT passes from distance 2^S_ up to 2^TOP.
+template <typename T, int R, int S_, int TOP, int LOGS, int LOGN, int NT, typename TWS>
+__device__ __forceinline__ void lds_inv_ct(typename C2<T>::type* x, TWS tw)
+{
+    if constexpr (S_ < TOP) {
+        lds_pass_ct<T, R, false, true, S_, LOGS, LOGN, NT, TWS>(x, tw);
+        lds_barrier();
+        lds_inv_ct<T, R, S_ + R, TOP, LOGS, LOGN, NT, TWS>(x, tw);
+    }
+}
+
+// The innermost Kb stages of both transforms and the product with the
+// spectrum (lds_turn with fused products).
+template <typename T, int Kb, int LOGS, int LOGN, int NT>
+__device__ __forceinline__ void lds_turn_ct(typename C2<T>::type* x, const typename C2<T>::type* __restrict__ Hbr)
+{
+    using V = typename C2<T>::type;
+    constexpr int G = 1 << Kb, items = (1 << LOGN) >> Kb;
+    static_assert(items % NT == 0, "whole items per thread");
+    const V* const none = nullptr;   // the constant rotations need no table
+#pragma unroll 1
+    for (int q0 = 0; q0 < items; q0 += NT) {
+        const int e0 = (q0 + (int)threadIdx.x) << Kb;
+        V* xb = x + lds_pad(e0);
+        V v[G];
+#pragma unroll
+        for (int m = 0; m < G; ++m) v[m] = xb[lds_step(m, 0)];
+        butterflies<T, Kb, true, false, true, true>(v, 0, 0, LOGS, none);
+#pragma unroll
+        for (int m = 0; m < G; ++m) v[m] = cmul_f(v[m], Hbr[e0 + m]);
+        butterflies<T, Kb, false, true, true, true>(v, 0, 0, LOGS, none);
+#pragma unroll
+        for (int m = 0; m < G; ++m) xb[lds_step(m, 0)] = v[m];
+    }
+}
+
+// Columns of 2^LS samples, 2^(LS - KP) threads, KP register stages, radix-2^R
+// LDS passes, Kb turn stages; NH LDS phases (as k_fft_filter_lds); PF: load
+// the next column during the inverse stages (one whole-column phase only).
+template <typename T, int LS, int KP, int Kb, int R, int NH, bool PF>
+__global__ __launch_bounds__(1 << (LS - KP), 1) void k_fft_filter_pc(const typename C2<T>::type* __restrict__ in,
+                                                                    typename C2<T>::type* __restrict__ out,
+                                                                    const typename C2<T>::type* __restrict__ Hbr,
+                                                                    const typename C2<T>::type* __restrict__ tw,
+                                                                    int64_t ncols)
+{
+    using V = typename C2<T>::type;
+    constexpr int S = 1 << LS, NT = S >> KP, P = 1 << KP, PH = P / NH, TOP = LS - KP;
+    constexpr int LNH = NH == 2 ? 1 : 0, LOGN = LS - LNH;
+    static_assert(LS == 14, "the two-level LDS twiddle table is sized for 2^14");
+    static_assert((TOP - Kb) % R == 0 && Kb >= 1 && Kb <= R && R <= 5, "pass structure");
+    static_assert(NH == 1 || (NH == 2 && !PF), "a prefetch needs the whole column in one phase");
+    extern __shared__ unsigned char smem[];
+    V* x = reinterpret_cast<V*>(smem);
+    V* lt = x + (S / NH + S / NH / 32 + 1);
+    const int t = threadIdx.x;
+    for (int k = t; k < 192; k += NT) lt[k] = k < 128 ? tw[k << 6] : tw[k - 128];
+    const TwLds<V> ltw{lt};
+    V* xt = x + lds_pad(t);   // the thread's elements t + i NT: xt[lds_step(i, TOP)]
+    int64_t col = blockIdx.x;
+    V v[P];
+    if (PF && col < ncols) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) v[i] = in[col * S + t + i * NT];
+    }
+    for (; col < ncols; col += gridDim.x) {
+        if (!PF) {
+#pragma unroll
+            for (int i = 0; i < P; ++i) v[i] = in[col * S + t + i * NT];
+        }
+        lds_barrier();   // the table, and the previous column's last LDS reads
+        butterflies<T, KP, true, false, false, true, TwLds<V>>(v, t, TOP, LS, ltw);
+        V vn[PF ? P : 1];
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            if (h) lds_barrier();
+#pragma unroll
+            for (int i = 0; i < PH; ++i) xt[lds_step(i, TOP)] = v[h * PH + i];
+            lds_barrier();
+            lds_fwd_ct<T, Kb, R, TOP, LS, LOGN, NT, TwLds<V>>(x, ltw);
+            lds_turn_ct<T, Kb, LS, LOGN, NT>(x, Hbr + h * (S >> LNH));
+            lds_barrier();
+            if constexpr (PF) {
+                const int64_t nc = col + gridDim.x;
+                if (nc < ncols) {
+#pragma unroll
+                    for (int i = 0; i < P; ++i) vn[i] = in[nc * S + t + i * NT];
+                }
+            }
+            lds_inv_ct<T, R, Kb, TOP, LS, LOGN, NT, TwLds<V>>(x, ltw);
+#pragma unroll
+            for (int i = 0; i < PH; ++i) v[h * PH + i] = xt[lds_step(i, TOP)];
+        }
+        butterflies<T, KP, false, true, false, true, TwLds<V>>(v, t, TOP, LS, ltw);
+        const T inv = (T)1 / (T)S;
+#pragma unroll
+        for (int i = 0; i < P; ++i) out[col * S + t + i * NT] = V{v[i].x * inv, v[i].y * inv};
+        if constexpr (PF) {
+#pragma unroll
+            for (int i = 0; i < P; ++i) v[i] = vn[i];
+        }
+    }
 }
 
 // Columns of length S = 2n above the LDS budget run as two independent
@@ -444,21 +679,22 @@ static hipError_t launch_lds_filter(typename C2<T>::type* data, const typename C
     return hipGetLastError();
 }
 
-// A lean variant of the filter: NH LDS phases, radix-2^R LDS passes, sized
-// for MINW waves per SIMD, up to NT threads.
-template <typename T, int KP, int Kb, int NH, int R, int MINW, int NT>
-static hipError_t launch_lds_filter_lean(typename C2<T>::type* data, const typename C2<T>::type* Hbr,
-                                         const typename C2<T>::type* tw, int logS, int64_t ncols, hipStream_t st,
-                                         int halves)
+// One workgroup per CU of the persistent filter (the LDS holds one column).
+template <typename T, int LS, int KP, int Kb, int R, int NH, bool PF>
+static hipError_t launch_filter_pc(typename C2<T>::type* data, const typename C2<T>::type* Hbr,
+                                   const typename C2<T>::type* tw, int64_t ncols, hipStream_t st)
 {
-    const int S = 1 << logS;
-    if ((S >> KP) > NT || logS - KP < 1 || (logS - KP - Kb) % R != 0) return hipErrorInvalidValue;
-    const size_t lb = lds_bytes<T>(S / NH);
-    hipError_t e = hipFuncSetAttribute((const void*)k_fft_filter_lds<T, KP, Kb, NH, R, MINW, NT>,
+    using V = typename C2<T>::type;
+    const size_t lb = lds_bytes<T>((1 << LS) / NH) + 192 * sizeof(V);
+    hipError_t e = hipFuncSetAttribute((const void*)k_fft_filter_pc<T, LS, KP, Kb, R, NH, PF>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_fft_filter_lds<T, KP, Kb, NH, R, MINW, NT>), dim3((unsigned)ncols), dim3(S >> KP), lb, st,
-                       data, data, Hbr, tw, logS, halves);
+    int dev = 0, cus = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    const unsigned grid = (unsigned)std::min<int64_t>(ncols, std::max(cus, 1));
+    hipLaunchKernelGGL((k_fft_filter_pc<T, LS, KP, Kb, R, NH, PF>), dim3(grid), dim3(1 << (LS - KP)), lb, st, data,
+                       data, Hbr, tw, ncols);
     return hipGetLastError();
 }
 
@@ -516,12 +752,12 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
             }
         }
     };
-    // 16384 complex128 samples (256 KiB) go through LDS in two halves with
-    // radix-8 passes in one 1024-thread workgroup per CU: one pass over HBM
-    // instead of the split path's three (the kernel spills 224 B per lane at
-    // its 128 VGPRs, radix-16 passes 472 B).
-    const bool lean = std::is_same<T, double>::value && logS == 14;
-    if (lean || (size_t)S * sizeof(V) <= kFftLdsMax) {
+    // 16384-sample columns run in the persistent filter: complex64 with the
+    // whole column in LDS and the next column prefetched, complex128 (256 KiB,
+    // twice the LDS budget) half a column at a time with 512 threads of 32
+    // samples (one pass over HBM instead of the split path's three).
+    const bool pc = logS == 14;
+    if (pc || (size_t)S * sizeof(V) <= kFftLdsMax) {
         // S / 2^KP <= 1024 threads: S <= 16384 (complex64) / 8192 (complex128)
         if (!tmp) return hipErrorInvalidValue;
         const V* tw = tw_given;
@@ -529,8 +765,11 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_bitrev_copy<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if constexpr (std::is_same<T, double>::value) {
-            if (lean) return launch_lds_filter_lean<T, 4, 1, 2, 3, 4, 1024>(data, tmp, tw, logS, ncols, st, 0);
+        if (pc) {
+            if constexpr (std::is_same<T, double>::value)
+                return launch_filter_pc<T, 14, 5, 3, 3, 2, false>(data, tmp, tw, ncols, st);
+            else
+                return launch_filter_pc<T, 14, 4, 2, 4, 1, true>(data, tmp, tw, ncols, st);
         }
         return lds(logS, tmp, tw, ncols, 0);
     }

@@ -149,6 +149,31 @@ def test_range_compress_fp32(bsar, ctx, oracle, S):
     assert np.abs(gpu - want).max() <= 2e-5 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_range_compress_16384_many_columns(bsar, ctx, form):
+    """16384-sample columns run in one persistent workgroup per CU that loops
+    over the columns (and, complex64, loads the next column while finishing
+    the current one): 601 columns, more than the GPU's CUs and not a multiple
+    of them, against numpy's transform column by column."""
+    rng = np.random.default_rng(601)
+    S, N = 16384, 601
+    sig = rng.normal(size=(N, S)) + 1j * rng.normal(size=(N, S))
+    ref = rng.normal(size=S) + 1j * rng.normal(size=S)
+    if form == "fp64":
+        b = bsar.DeviceBuffer.from_array(sig)
+        ctx.range_compress_device(b, S, N, ref, conj=True, precision=bsar.FP64)
+        gpu, base, tol = b.read((N, S), np.complex128), sig, 1e-13
+    else:
+        s32 = sig.astype(np.complex64)
+        b = bsar.DeviceBuffer.from_array(s32)
+        ctx.range_compress_device(b, S, N, ref, conj=True, precision=bsar.MIXED)
+        gpu, base, tol = b.read((N, S), np.complex64), s32.astype(np.complex128), 2e-6
+    b.close()
+    exact = np.fft.ifft(np.fft.fft(base, axis=1) * np.conj(np.fft.fft(ref))[None, :], axis=1)
+    err = np.abs(gpu - exact).max(axis=1) / np.abs(exact).max()
+    assert err.max() <= tol, (int(err.argmax()), float(err.max()))
+
+
 @pytest.mark.parametrize("n", [1, 7, 12])
 def test_range_compress_non_power_of_two(ctx, golden, n):
     """The reference pads to the next power of two and keeps the first n
