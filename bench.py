@@ -39,7 +39,16 @@ ROOT = Path(__file__).resolve().parent
 PKG_DIR = ROOT / "c-_bistatic_sar_amd"
 
 P64 = 78.6e12     # MI355X FP64 vector peak, FLOP/s
-P32 = 157.3e12    # MI355X FP32 vector peak, FLOP/s
+P32 = 157.3e12    # MI355X FP32 vector peak, FLOP/s (reached without packed FP32:
+                  # a wave64 FP32 FMA issues every 2 cycles, FP64 every 4;
+                  # c-_bistatic_sar_amd/tools/valu_rates.hip)
+
+
+def mix_peak(e64: float, e32: float) -> float:
+    """Vector peak for a kernel executing e64 FP64 and e32 FP32 flop per unit:
+    the rate at which both precisions' peaks would finish that mix,
+    (e64 + e32) / (e64 / P64 + e32 / P32)."""
+    return (e64 + e32) / (e64 / P64 + e32 / P32)
 
 # Algorithmic flop per voxel-pulse (add/sub/mul/div/sqrt = 1, FMA = 2,
 # sin = cos = 1; loop-invariant terms hoisted):
@@ -120,12 +129,12 @@ def roofline(rep: dict, n: int, form: str, flops_prof: str = PROFILE_FLOPS, traf
     The executed flop per voxel-pulse comes from rocprofv3 counters
     (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_{F64,F32}: 64 lanes per wave
     instruction, FMA = 2) over the same image, committed in
-    profiles/valu_flops_bp_large.md.  peak = 78.6 TFLOP/s: MI355X's FP64 vector
-    peak, which is also the FP32 ceiling of this build (packed FP32
-    instructions are disabled, so FP32 issues 64 FMA lanes per CU per clock,
-    as FP64 does).  effective_tflops counts the reference algorithm's flop
-    instead (709 exact candidates per refraction solve), which the screened
-    search does not execute."""
+    profiles/valu_flops_bp_large.md.  peak = MI355X's published vector peaks
+    (78.6 TFLOP/s FP64, 157.3 FP32) combined for the kernel's executed mix
+    (mix_peak), so frac is the fraction of the time both peaks would need.
+    effective_tflops counts the reference algorithm's flop instead (709 exact
+    candidates per refraction solve), which the screened search does not
+    execute."""
     f64, f32 = (flops_mixed if form == "mixed" else flops_fp64)(n)
     launches = max(int(rep["launches"]) - 1, 1) if rep["launches"] > 1 else 1
     t_launch = rep["seconds"] / launches
@@ -141,7 +150,8 @@ def roofline(rep: dict, n: int, form: str, flops_prof: str = PROFILE_FLOPS, traf
            "voxel_pulses_per_launch": vp_launch}
     if e64 is not None and e32 is not None:
         ach = (e64 + e32) * vp_launch / t_launch
-        out.update(achieved=round(ach / 1e12, 3), frac=round(ach / P64, 4),
+        pk = mix_peak(e64, e32)
+        out.update(achieved=round(ach / 1e12, 3), peak=round(pk / 1e12, 2), frac=round(ach / pk, 4),
                    executed_flop_per_voxel_pulse={"fp64": e64, "fp32": e32,
                                                   "source": f"profiles/{flops_prof}",
                                                   "commit": fcommit})
@@ -156,8 +166,8 @@ def roofline(rep: dict, n: int, form: str, flops_prof: str = PROFILE_FLOPS, traf
     eff = (f64 + f32) * vp_launch / t_launch
     out["effective_tflops"] = round(eff / 1e12, 3)
     out["reference_flop_per_voxel_pulse"] = {"fp64": f64, "fp32": f32}
-    out["peak_note"] = ("FP64 vector peak; also the FP32 ceiling here (no packed FP32: 64 FMA lanes per CU per "
-                        "clock in both precisions)")
+    out["peak_note"] = ("published vector peaks 78.6 (FP64) and 157.3 (FP32) TFLOP/s combined for the executed "
+                        "FP64/FP32 mix; non-packed FP32 issues at twice the FP64 rate (tools/valu_rates.hip)")
     return out
 
 
@@ -468,6 +478,7 @@ def run_rough(args, pkg, scene) -> dict:
     e64, e32 = _num(rows.get("k_slant_range", {}), "fp64 flop/solve"), _num(rows.get("k_slant_range", {}),
                                                                              "fp32 flop/solve")
     executed = None if e64 is None or e32 is None else value * (e64 + e32)
+    pk = P64 if executed is None else mix_peak(e64, e32)
     return {"metric": "refraction solves/s (Fermat search over the DEM, FP64), both legs of every voxel-pulse",
             "value": round(value, 1), "unit": "solve/s", "n_gpus": 1, "steps": 1, "warmup": 0,
             "ms_per_step": round(dev_s * 1e3, 1), "higher_is_better": True, "scaling": "weak",
@@ -478,8 +489,9 @@ def run_rough(args, pkg, scene) -> dict:
                        "chunk_voxels": chunk},
             "wall_s": round(wall, 2),
             "roofline": {"bound": "valu",
-                         "achieved": None if executed is None else round(executed / 1e12, 3), "peak": P64 / 1e12,
-                         "unit": "TFLOP/s", "frac": None if executed is None else round(executed / P64, 4),
+                         "achieved": None if executed is None else round(executed / 1e12, 3),
+                         "peak": round(pk / 1e12, 2),
+                         "unit": "TFLOP/s", "frac": None if executed is None else round(executed / pk, 4),
                          "traffic": None, "kernel": "k_slant_range",
                          "executed_flop_per_solve": None if executed is None else {
                              "fp64": e64, "fp32": e32, "source": "profiles/valu_flops_slant_rough.md",
@@ -534,6 +546,7 @@ def run_wide(args, pkg, scene) -> dict:
         rc_err = float(np.abs(comp32 - comp64).max() / np.abs(comp64).max())
         bytes32 = 2.0 * N * S * 8
         gbs = bytes32 / rc32["seconds"] / 1e9
+        gbs64 = 2.0 * bytes32 / rc64["seconds"] / 1e9
         log(f"range compression FP32: {rc32['seconds'] * 1e3:.2f} ms ({gbs:.0f} GB/s), FP64: "
             f"{rc64['seconds'] * 1e3:.2f} ms; FP32 vs FP64 {20 * np.log10(max(rc_err, 1e-300)):.1f} dB")
         t1 = time.perf_counter()
@@ -568,6 +581,8 @@ def run_wide(args, pkg, scene) -> dict:
            "range_compression": {"fp32_ms": round(rc32["seconds"] * 1e3, 3), "fp64_ms": round(rc64["seconds"] * 1e3, 3),
                                  "fp32_gb_s": round(gbs, 1), "roofline_frac": round(gbs * 1e9 / 8e12, 4),
                                  "algorithmic_bytes": bytes32, "bound": "hbm",
+                                 "fp64_gb_s": round(gbs64, 1), "fp64_roofline_frac": round(gbs64 * 1e9 / 8e12, 4),
+                                 "fp64_algorithmic_bytes": 2.0 * bytes32,
                                  "fp32_vs_fp64_db": round(20 * np.log10(max(rc_err, 1e-300)), 2)},
            "fp64_subset": {"voxels": v64, "value": round(v64 * N / rep64["seconds"], 1),
                            "seconds": round(rep64["seconds"], 2)},
