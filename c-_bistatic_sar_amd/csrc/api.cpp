@@ -177,6 +177,12 @@ BpArgs make_bp_args(const bsar_dataset* ds, int K, const double* dg, int nx, int
     a.dem = ds->dem;
     a.traj = ds->traj;
     a.traj.n = K;                                  // first K pulses; R1 means over those
+    a.r1 = ds->r1;
+    if (K > 0 && K <= ds->n) {                     // antenna bounds of the first K pulses
+        a.r1.bsq = ds->r1_bsq[K - 1];
+        a.r1.zlo = ds->r1_zlo[K - 1];
+        a.r1.zhi = ds->r1_zhi[K - 1];
+    }
     a.rc = ds->rc;
     a.rc32 = ds->rc32;
     a.ndata = ds->ndata;
@@ -527,12 +533,48 @@ int bsar_dataset_create(bsar_ctx* c, const bsar_params* p, const bsar_dem* dem, 
     if (st) { bsar_dataset_destroy(ds); return st; }
     ds->htx.assign(tx, tx + 3 * (size_t)n);
     ds->hrx.assign(rx, rx + 3 * (size_t)n);
-    std::vector<double> soa(6 * (size_t)std::max(n, 1), 0.0);
+    // [tx ty tz rx ry rz] followed by the R1 sums' expansion terms (R1Geo):
+    // Tx and Rx each as b = A - o per axis and c = |b|^2.  The origin o (the
+    // first pulse's Tx/Rx midpoint) and the bounds kept per prefix of pulses
+    // make an image over the first K pulses sum exactly the terms a dataset
+    // holding only those K pulses sums.
+    std::vector<double> soa(14 * (size_t)std::max(n, 1), 0.0);
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) {
             soa[(size_t)k * n + i] = tx[3 * i + k];
             soa[(size_t)(3 + k) * n + i] = rx[3 * i + k];
         }
+    double o[3] = {0.0, 0.0, 0.0};
+    if (n > 0)
+        for (int k = 0; k < 3; ++k) o[k] = (tx[k] + rx[k]) / 2.0;
+    try {
+        ds->r1_bsq.resize(n);
+        ds->r1_zlo.resize(n);
+        ds->r1_zhi.resize(n);
+    } catch (const std::bad_alloc&) {
+        bsar_dataset_destroy(ds);
+        return set_error(c, BSAR_ERR_NOMEM, "host allocation");
+    }
+    double bsq = 0.0, zlo = INFINITY, zhi = -INFINITY;
+    for (int i = 0; i < n; ++i) {
+        for (int set = 0; set < 2; ++set) {
+            const double* A = set ? rx : tx;
+            double* B = soa.data() + (size_t)(6 + 4 * set) * n;
+            double cc = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const double b = A[3 * i + k] - o[k];
+                B[(size_t)k * n + i] = b;
+                cc += b * b;
+            }
+            B[(size_t)3 * n + i] = cc;
+            bsq = std::max(bsq, cc);   // a NaN term leaves bsq; then Q is NaN too and the sums go direct
+            zlo = std::min(zlo, A[3 * i + 2]);
+            zhi = std::max(zhi, A[3 * i + 2]);
+        }
+        ds->r1_bsq[i] = bsq * (1.0 + 1e-12);   // above every stored c despite their roundings
+        ds->r1_zlo[i] = zlo;
+        ds->r1_zhi[i] = zhi;
+    }
     hipError_t e = hipMalloc((void**)&ds->dtraj, soa.size() * sizeof(double));
     if (e == hipSuccess) e = hipMemcpy(ds->dtraj, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) { bsar_dataset_destroy(ds); return hip_fail(c, e, "trajectory upload"); }
@@ -543,6 +585,12 @@ int bsar_dataset_create(bsar_ctx* c, const bsar_params* p, const bsar_dem* dem, 
     ds->traj.ry = ds->dtraj + 4 * (size_t)n;
     ds->traj.rz = ds->dtraj + 5 * (size_t)n;
     ds->traj.n = n;
+    ds->r1.t = ds->dtraj + 6 * (size_t)n;
+    ds->r1.r = ds->dtraj + 10 * (size_t)n;
+    ds->r1.stride = n;
+    ds->r1.ox = o[0];
+    ds->r1.oy = o[1];
+    ds->r1.oz = o[2];
     ds->bytes = (int64_t)soa.size() * 8 + (int64_t)(ds->dem.nx + ds->dem.ny + ds->dem.ny * ds->dem.zs) * 8 +
                 (int64_t)ds->dem.ny * ds->dem.zs * 4;
     *out = ds;

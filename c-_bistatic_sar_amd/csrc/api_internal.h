@@ -44,9 +44,11 @@ struct bsar_dataset {
     bsar::DemDev dem{};
     double zmin = 0.0, zmax = 0.0;   // DEM height range (0 when empty)
     // antennas (SoA) on device + host copy
-    double* dtraj = nullptr;         // 6 * n doubles
+    double* dtraj = nullptr;         // 14 * n doubles: positions, then R1Geo's terms
     std::vector<double> htx, hrx;    // [n][3]
     bsar::Traj traj{};
+    bsar::R1Geo r1{};                // expansion terms of the plain form's R1 sums (in dtraj)
+    std::vector<double> r1_bsq, r1_zlo, r1_zhi;   // their antenna bounds over pulses 0..i
     // upsampled range-compressed data
     double2* rc = nullptr;
     float2* rc32 = nullptr;

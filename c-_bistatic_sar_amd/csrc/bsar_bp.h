@@ -25,11 +25,24 @@ __device__ __forceinline__ double sqrt_faithful(double a)
     return __builtin_fma(d, h, s1);
 }
 
+// Conditioning limit of the expanded squared distances in r1_mean_fp64.
+constexpr double kR1Cond = 32.0;
+
 // Mean distance from the n antennas (ax, ay, az) to q (proc:518-548),
-// summed in antenna order with faithful square roots.  Each squared distance
-// is f0^2 + f1^2 + f2^2 with the last two terms fused (f2^2 + (f1^2 + f0^2)
-// as two fmas: at most one rounding fewer than the reference's three, so
-// within 1 ulp of its value; 14 instructions per antenna instead of 16).
+// summed in antenna order with faithful square roots.
+//
+// Squared distances.  When q is well separated from the antennas relative
+// to their spread -- bsq + |q - o|^2 <= kR1Cond gap^2, gap = q's vertical
+// distance from the antennas' height range, so every squared distance a is
+// at least gap^2 -- each one is expanded about the origin o of g (R1Geo):
+// a = c - 2 b.(q - o) + |q - o|^2 as one add and three fmas (12 instructions
+// per antenna with the root and the sum).  Every intermediate is at most
+// 2 (bsq + |q - o|^2), so the roundings (these four, |q - o|^2, c and the
+// rounded b and q - o) stay below 11 eps (bsq + |q - o|^2) <= 352 eps a:
+// 4e-14 relative on a, 2e-14 on the distance.  Otherwise (antennas spread
+// wide against their height above q, q among them, non-finite values) the
+// direct form: f0^2 + f1^2 + f2^2 with the last two terms fused (within 1 ulp
+// of the reference's value; 14 instructions per antenna).
 // When a squared distance is zero or subnormal, sqrt_faithful's NaN makes
 // the sum NaN and it is redone with correctly rounded roots, which also
 // gives the reference's NaN / infinite mean for NaN / infinite distances
@@ -37,14 +50,31 @@ __device__ __forceinline__ double sqrt_faithful(double a)
 // [2^-1022, 2^-1000) -- distances below 1e-150 m -- are outside
 // sqrt_faithful's range and may be off by a few ulp.
 __device__ __forceinline__ double r1_mean_fp64(const double* __restrict__ ax, const double* __restrict__ ay,
-                                               const double* __restrict__ az, int n, double qx, double qy, double qz)
+                                               const double* __restrict__ az, const R1Geo& g,
+                                               const double* __restrict__ gb, int n, double qx, double qy, double qz)
 {
     double s = 0.0;
+    const double ux = qx - g.ox, uy = qy - g.oy, uz = qz - g.oz;
+    const double Q = __builtin_fma(uz, uz, __builtin_fma(uy, uy, ux * ux));
+    const double gap = fmax(fmax(g.zlo - qz, qz - g.zhi), 0.0);
+    if (g.bsq + Q <= kR1Cond * (gap * gap)) {
+        const double* __restrict__ bx = gb;
+        const double* __restrict__ by = gb + g.stride;
+        const double* __restrict__ bz = gb + 2 * (int64_t)g.stride;
+        const double* __restrict__ bc = gb + 3 * (int64_t)g.stride;
+        const double mx = -2.0 * ux, my = -2.0 * uy, mz = -2.0 * uz;
 #pragma unroll 4
-    for (int j = 0; j < n; ++j) {
-        const double f0 = ax[j] - qx, f1 = ay[j] - qy, f2 = az[j] - qz;
-        const double a = __builtin_fma(f2, f2, __builtin_fma(f1, f1, f0 * f0));
-        s += sqrt_faithful(a);
+        for (int j = 0; j < n; ++j) {
+            const double a = __builtin_fma(bx[j], mx, __builtin_fma(by[j], my, __builtin_fma(bz[j], mz, bc[j] + Q)));
+            s += sqrt_faithful(a);
+        }
+    } else {
+#pragma unroll 4
+        for (int j = 0; j < n; ++j) {
+            const double f0 = ax[j] - qx, f1 = ay[j] - qy, f2 = az[j] - qz;
+            const double a = __builtin_fma(f2, f2, __builtin_fma(f1, f1, f0 * f0));
+            s += sqrt_faithful(a);
+        }
     }
     if (!(s == s)) {
         s = 0.0;
@@ -80,8 +110,8 @@ __device__ __forceinline__ void bp_fp64_thread(const DemDev& d, const BpArgs& a,
         const double R2t = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:496-505
         e0 = Px - qbx; e1 = Py - qby; e2 = Pz - qbz;
         const double R2r = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);                                   // proc:507-516
-        const double R1t = r1_mean_fp64(a.traj.tx, a.traj.ty, a.traj.tz, n, qax, qay, qaz);              // proc:518-532
-        const double R1r = r1_mean_fp64(a.traj.rx, a.traj.ry, a.traj.rz, n, qbx, qby, qbz);              // proc:534-548
+        const double R1t = r1_mean_fp64(a.traj.tx, a.traj.ty, a.traj.tz, a.r1, a.r1.t, n, qax, qay, qaz); // proc:518-532
+        const double R1r = r1_mean_fp64(a.traj.rx, a.traj.ry, a.traj.rz, a.r1, a.r1.r, n, qbx, qby, qbz); // proc:534-548
         const double path = n1 * R1t + n2 * R2t + n2 * R2r + n1 * R1r;
         const double tv = a.inv_c * path;                                                                // proc:567
         const double rngBin = 1.0 + (tv * a.fs4);                                                        // proc:571

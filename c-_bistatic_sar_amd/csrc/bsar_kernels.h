@@ -61,6 +61,20 @@ struct Traj {
     int n;
 };
 
+// Antennas about a local origin o for the plain form's R1 sums: per set
+// (t = Tx, r = Rx) four arrays of stride entries, b = A - o per axis and
+// c = |b|^2, so |A - q|^2 = c - 2 b.(q - o) + |q - o|^2 takes one add and
+// three fmas per antenna.  bsq bounds every c from above; [zlo, zhi] is the
+// antennas' height range.
+struct R1Geo {
+    const double* t;
+    const double* r;
+    int stride;
+    double ox, oy, oz;
+    double bsq;
+    double zlo, zhi;
+};
+
 // One back-projection launch: voxels [vox0, vox0 + nvox) of an nx*ny*nz
 // grid (x fastest), pulse groups g = 0..groups-1 each covering pulses
 // [g*glen, min((g+1)*glen, n)), of which this launch adds the window
@@ -68,6 +82,7 @@ struct Traj {
 struct BpArgs {
     DemDev dem;
     Traj traj;
+    R1Geo r1;
     const double2* rc;      // upsampled range-compressed data [ndata][S4]
     const float2* rc32;     // same in FP32 (mixed form), may be null
     int ndata, S4;
