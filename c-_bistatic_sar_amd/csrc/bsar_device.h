@@ -176,7 +176,10 @@ __device__ __forceinline__ double dem_interp(const DemDev& d, double x, double y
 // come from a position within epsy (~1e-11 m) of the candidate's and an FP32
 // column weight (axis_cell_inside: 5 u relative, i.e. at most 10 u |Z|max more
 // on a height, (n1 + n2) 10 u |Z|max on the estimate), which the 1.6x leaves
-// room for (75 u of the 1e-5).  A candidate is
+// room for (75 u of the 1e-5).  The y offsets are FP32: dy1 = py - y1 (one
+// rounding) and dy2 = (y2 - y1) - dy1 (three), so dy2 is off by at most
+// 2u (|y2 - y1| + |dy2|): the relative part is within the 5x of kRho32, the
+// absolute part adds (n1 + n2) 2^-22 |y2 - y1| to D32 (make_leg).  A candidate is
 // dropped only when est > M (1 + 2 kRho32) + 2 D32, which puts its exact
 // value above M; M itself only takes exact values or est upper bounds
 // (est + D32)(1 + 3 kRho32).  Any candidate dropped by a bound or an estimate
@@ -234,7 +237,7 @@ __device__ __forceinline__ Leg make_leg(const DemDev& d, double x1, double y1, d
     const double g2 = fmax(fmax(d.zlo - z2, z2 - d.zhi), 0.0);
     L.g1sq = g1 * g1;
     L.g2sq = g2 * g2;
-    L.D = (fabs(n1) + fabs(n2)) * 1e-5 * (d.zabs + fabs(z1) + fabs(z2) + 1.0);
+    L.D = (fabs(n1) + fabs(n2)) * (1e-5 * (d.zabs + fabs(z1) + fabs(z2) + 1.0) + 0x1p-22 * fabs(y2 - y1));
     L.z1f = (float)z1; L.z2f = (float)z2; L.n1f = (float)n1; L.n2f = (float)n2;
     // FP32 estimates need every magnitude well inside FP32 range
     const double big = 1e15;
@@ -336,6 +339,102 @@ __device__ __forceinline__ void local_gaps(const DemDev& d, double z1, double z2
     g2sq = g2 * g2;
 }
 
+// Relative margin of window_gaps' FP32 height range (|Z|max units): the
+// heights are formed from FP32 node heights and differences (one rounding
+// each), an FP32 weight (the rounded FP64 offset times v_rcp_f32 of the
+// rounded interval width: about 3 ulp) and two fmas, at most ~8 ulp of
+// |Z|max (8 * 2^-24 = 4.8e-7) from the exact surface; the margin is 8x that.
+constexpr double kGapMargin = 4e-6;
+
+// The squared vertical gaps of local_gaps from the heights at the window
+// itself instead of whole DEM cells.  On a strictly increasing axis the
+// lookup returns the bilinear surface of the cells the window meets (for
+// positions strictly inside the axis range), which is linear along each
+// axis inside a cell; over [xa, xb] x [ya, yb] its extremes therefore lie
+// at the corners of the window's pieces in each cell: the x positions xa,
+// the DEM columns strictly inside, xb, against the y positions ya, the rows
+// strictly inside, yb.  Those (up to 4 x 4) heights are formed in FP32 from
+// the (height, x difference) pairs (kPairs) or the FP32 heights, and the
+// range is widened by kGapMargin |Z|max.  Windows that reach the axis ends
+// or span more than three intervals per axis keep local_gaps' node range.
+// NaN heights are skipped by fmin/fmax: a candidate whose lookup meets one
+// gets a NaN path, which never wins.
+template <bool kPairs>
+__device__ __forceinline__ void window_gaps(const DemDev& d, double z1, double z2, double xa, double xb,
+                                            double ya, double yb, double& g1sq, double& g2sq)
+{
+    int i0, i1, j0, j1;
+    window_nodes(d, xa, xb, ya, yb, i0, i1, j0, j1);
+    // window_nodes: X[i0] <= xa, xb <= X[i1], Y[j0] <= ya, yb <= Y[j1]; strictly
+    // inside the axis range i0 < i1 and j0 < j1 whenever xa < xb and ya < yb
+    const bool tight = xa > d.x0 && xb < d.xn && ya > d.y0 && yb < d.yn && i0 < i1 && j0 < j1 &&
+                       i1 - i0 <= 3 && j1 - j0 <= 3;
+    if (!tight) {
+        local_gaps(d, z1, z2, xa, xb, ya, yb, g1sq, g2sq);
+        return;
+    }
+    auto pair = [&](int j, int i) -> float2 {
+        if (kPairs) return d.Zp[j * d.zs + i];
+        const float a = d.Zf[j * d.zs + i];
+        return make_float2(a, d.Zf[j * d.zs + i + 1] - a);
+    };
+    auto weight = [](double x, double lo, double hi) {
+        const float w = (float)(x - lo) * __builtin_amdgcn_rcpf((float)(hi - lo));
+        return __builtin_fminf(__builtin_fmaxf(w, 0.0f), 1.0f);
+    };
+    const float wa = weight(xa, d.X[i0], d.X[i0 + 1]);
+    const float wb = weight(xb, d.X[i1 - 1], d.X[i1]);
+    const float wya = weight(ya, d.Y[j0], d.Y[j0 + 1]);
+    const float wyb = weight(yb, d.Y[j1 - 1], d.Y[j1]);
+    const int lx = i1 - i0, ly = j1 - j0;   // 1..3 intervals per axis
+    // P[b][m]: node row j0 + b (clamped to j1) at the x positions xa, the
+    // inner columns i0 + 1 .. i1 - 1, xb (position m clamped to lx)
+    float P[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int j = j0 + min(b, ly);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int mm = min(m, lx);
+            if (mm == 0) {
+                const float2 p = pair(j, i0);
+                P[b][m] = __builtin_fmaf(p.y, wa, p.x);
+            } else if (mm == lx) {
+                const float2 p = pair(j, i1 - 1);
+                P[b][m] = __builtin_fmaf(p.y, wb, p.x);
+            } else {
+                P[b][m] = pair(j, i0 + mm).x;
+            }
+        }
+    }
+    float lo = __builtin_inff(), hi = -__builtin_inff();
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        // y positions: ya (rows j0, j0 + 1), the inner rows, yb (rows j1 - 1, j1)
+        const float ha = __builtin_fmaf(P[1][m] - P[0][m], wya, P[0][m]);
+        const float pb0 = ly == 1 ? P[0][m] : (ly == 2 ? P[1][m] : P[2][m]);
+        const float pb1 = ly == 1 ? P[1][m] : (ly == 2 ? P[2][m] : P[3][m]);
+        const float hb = __builtin_fmaf(pb1 - pb0, wyb, pb0);
+        lo = __builtin_fminf(lo, __builtin_fminf(ha, hb));
+        hi = __builtin_fmaxf(hi, __builtin_fmaxf(ha, hb));
+        if (ly >= 2) {
+            lo = __builtin_fminf(lo, P[1][m]);
+            hi = __builtin_fmaxf(hi, P[1][m]);
+        }
+        if (ly == 3) {
+            lo = __builtin_fminf(lo, P[2][m]);
+            hi = __builtin_fmaxf(hi, P[2][m]);
+        }
+    }
+    if (!(hi >= lo)) return;   // every height NaN: keep the global range
+    const double mg = kGapMargin * d.zabs;
+    const double dlo = (double)lo - mg, dhi = (double)hi + mg;
+    const double g1 = fmax(fmax(dlo - z1, z1 - dhi), 0.0);
+    const double g2 = fmax(fmax(dlo - z2, z2 - dhi), 0.0);
+    g1sq = g1 * g1;
+    g2sq = g2 * g2;
+}
+
 // Exact optical path of candidate (px, py) in the reference's operation order
 // (raw_data_processing.cpp:525-534 with interpolateDEMElevation :615-680);
 // i0, di, wx are px's x-interval terms, dxa = (px - x1)^2, dxb = (x2 - px)^2.
@@ -367,6 +466,52 @@ __device__ __forceinline__ double exact_opt(const DemDev& d, const Leg& L, bool 
     return L.n1 * d1 + L.n2 * d2;
 }
 
+// Lower bound of the path through any point of a row (fixed x) of a window:
+//   h(y) = n1 sqrt(a1 + (y - c1)^2) + n2 sqrt(a2 + (y - c2)^2),  y in [0, w],
+// with a1 = dx1^2 + g1^2, a2 = dx2^2 + g2^2 (squared x offsets of the row from
+// the end points plus their squared vertical gaps to the window's height
+// range) and c1, c2 the end points' y in the window's frame.  h is convex, so
+// its tangent at any y~ in [0, w] lies below it there:
+//   min h >= h(y~) + min(h'(y~) (0 - y~), h'(y~) (w - y~)),
+// which is tight when y~ is near the minimiser: y~ is one Newton step on h'
+// from the straight line's crossing of the row (clamped to [0, w]).  FP32
+// with v_rsq_f32: h within ~6 ulp relative, h' within ~8 ulp of n1 + n2
+// absolute; the result is deflated by kDefl and by `slack` (row_slack:
+// 1e-5 (n1 + n2) (w + 1) for those and for candidates up to 1e-5 outside
+// [0, w], plus 2^-22 (n1 |c1| + n2 |c2|) for c1, c2 rounded to FP32).
+// Non-finite or degenerate input gives NaN (no pruning).
+__device__ __forceinline__ float row_slack(float c1, float c2, float w, float n1, float n2)
+{
+    return 1e-5f * (n1 + n2) * (w + 1.0f) + 0x1p-22f * (n1 * __builtin_fabsf(c1) + n2 * __builtin_fabsf(c2));
+}
+
+__device__ __forceinline__ float row_bound2d(float a1, float a2, float c1, float c2, float w, float n1, float n2,
+                                             float slack)
+{
+    // start: the straight line between the end points, weighted by their
+    // distances from the row
+    const float r1 = __builtin_amdgcn_rsqf(a1), r2 = __builtin_amdgcn_rsqf(a2);
+    const float f = r2 * __builtin_amdgcn_rcpf(r1 + r2);   // sqrt(a1) / (sqrt(a1) + sqrt(a2))
+    float y = __builtin_fmaf(c2 - c1, f, c1);
+    y = __builtin_fminf(__builtin_fmaxf(y, 0.0f), w);
+    {
+        const float t1 = y - c1, t2 = y - c2;
+        const float q1 = __builtin_fmaf(t1, t1, a1), q2 = __builtin_fmaf(t2, t2, a2);
+        const float s1 = __builtin_amdgcn_rsqf(q1), s2 = __builtin_amdgcn_rsqf(q2);
+        const float u1 = n1 * s1, u2 = n2 * s2;
+        const float d1 = __builtin_fmaf(u1, t1, u2 * t2);                              // h'
+        const float d2 = __builtin_fmaf(u1 * a1, s1 * s1, (u2 * a2) * (s2 * s2));      // h''
+        y = __builtin_fminf(__builtin_fmaxf(y - d1 * __builtin_amdgcn_rcpf(d2), 0.0f), w);
+    }
+    const float t1 = y - c1, t2 = y - c2;
+    const float q1 = __builtin_fmaf(t1, t1, a1), q2 = __builtin_fmaf(t2, t2, a2);
+    const float s1 = __builtin_amdgcn_rsqf(q1), s2 = __builtin_amdgcn_rsqf(q2);
+    const float h = __builtin_fmaf(n1 * q1, s1, (n2 * q2) * s2);
+    const float d = __builtin_fmaf(n1 * t1, s1, (n2 * t2) * s2);
+    const float drop = __builtin_fminf(d * (0.0f - y), d * (w - y));
+    return __builtin_fmaf(h, (float)kDefl, drop) - slack;
+}
+
 template <int G, bool kSplit, bool kPairs, bool kSide, bool kReuse>
 __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, double R, double& bestX, double& bestY,
                                              double& minOpt)
@@ -384,7 +529,8 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     // vertical gaps for the bounds: the DEM's global height range, or the
     // range inside this level's window once the window is small
     double g1sq = L.g1sq, g2sq = L.g2sq;
-    if (G == 11 && !empty && xsorted && ysorted) local_gaps(d, L.z1, L.z2, minX, maxX, minY, maxY, g1sq, g2sq);
+    if (G == 11 && !empty && xsorted && ysorted)
+        window_gaps<kPairs>(d, L.z1, L.z2, minX - epsx, maxX + epsx, minY - epsy, maxY + epsy, g1sq, g2sq);
     const float g1f = (float)g1sq, g2f = (float)g2sq;
 
     // Probe: the centre candidate (the previous level's best) evaluated
@@ -420,8 +566,8 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     // Columns whose lower bound can reach M (bit t <-> column
     // center_out(JB + t), fixed for the part: M only decreases, so later rows
     // may estimate a column the bound would now drop) and their FP32 terms: Z
-    // element offsets of the rows j0 and j0 + dj, weight, squared y offsets.
-    // A dropped column keeps sy1 = +inf, which makes its estimate +inf (or
+    // element offsets of the rows j0 and j0 + dj, weight, y offset.
+    // A dropped column keeps dyf = +inf, which makes its estimate +inf (or
     // NaN), so the screening loop needs no per-candidate mask test.
     //
     // Side stop.  The one-axis bound is convex in the column (row) position,
@@ -437,21 +583,25 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     // SIMD, would spill the extra state.  kSide also ends a row's screening
     // loop early (below), which measured slower in the search-only kernels.
     const float slope = kSlope * (L.n1f + L.n2f);
+    const float dyL = (float)(L.y2 - L.y1);
     unsigned cdone = 0u;
     auto part = [&](auto jb, auto nj) {
     constexpr int JB = decltype(jb)::value, NJ = decltype(nj)::value;
-    int o0[NJ], o1[NJ];
-    float wyf[NJ], sy1[NJ], sy2[NJ];
+    // kPairs (DEMs staged in LDS, fewer than 2^16 pair slots): both row
+    // offsets in one word, o0 | o1 << 16; otherwise o0 and o1.  The squared y
+    // offsets come from dyf = py - y1 and dyL = y2 - y1 in FP32.
+    int o0[NJ];
+    int o1[kPairs ? 1 : NJ];
+    float wyf[NJ], dyf[NJ];
     unsigned colmask = 0u;
 #pragma unroll
     for (int t = 0; t < NJ; ++t) {
         const int kk = JB + t;
         const int j = center_out(kk, C);
         o0[t] = 0;
-        o1[t] = 0;
+        if (!kPairs) o1[t] = 0;
         wyf[t] = 0.0f;
-        sy1[t] = __builtin_inff();
-        sy2[t] = 0.0f;
+        dyf[t] = __builtin_inff();
         const unsigned side = (kk & 1) ? 2u : 1u;
         if (kSide && kk > 0 && (cdone & side)) continue;
         const double pya = __builtin_fma((double)j, stepY, minY);
@@ -481,12 +631,14 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
                 wf = (float)w;
             }
         }
-        o0[t] = j0 * d.zs;
-        o1[t] = (j0 + dj) * d.zs;
+        if (kPairs) {
+            o0[t] = j0 * d.zs | ((j0 + dj) * d.zs) << 16;
+        } else {
+            o0[t] = j0 * d.zs;
+            o1[t] = (j0 + dj) * d.zs;
+        }
         wyf[t] = wf;
-        const double dy1 = pya - L.y1, dy2 = L.y2 - pya;
-        sy1[t] = (float)(dy1 * dy1);
-        sy2[t] = (float)(dy2 * dy2);
+        dyf[t] = (float)(pya - L.y1);
     }
     // FP32 screening constants (rounded to FP32: well inside the 2 kRho32 and
     // D32 slack of the bounds)
@@ -520,11 +672,25 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         const unsigned colok = (JB == 0 && i == C) ? (colmask & ~1u) : colmask;
         if (!colok) continue;
         const double px = minX + i * spanX / (G - 1);
+        const double dx1 = px - L.x1, dx2 = L.x2 - px;
+        const double dxa = dx1 * dx1, dxb = dx2 * dx2;
+        const float dxaf = (float)dxa, dxbf = (float)dxb;
+        float Mf = (float)(M * (1.0 + 0x1p-22));   // >= M: an upper bound of the minimum
+        // Row bound over both axes (11-wide levels, once the one-axis bound
+        // kept the row): the least path through the row's x at the window's
+        // height range, row_bound2d below.  A row it prunes needs nothing
+        // more, not even its x lookup.
+        if (G == 11 && L.screen) {
+            // end points' y relative to the window's low edge (recomputed per
+            // row rather than kept live through the row loop)
+            const float wy1 = (float)(L.y1 - minY), wy2 = (float)(L.y2 - minY), wspan = (float)spanY;
+            if (row_bound2d(dxaf + g1f, dxbf + g2f, wy1, wy2, wspan, L.n1f, L.n2f,
+                            row_slack(wy1, wy2, wspan, L.n1f, L.n2f)) > Mf)
+                continue;
+        }
         int i0 = 0, di = 0;
         double wx = 0.0;
         if (!empty) axis_lookup(d.X, d.nx, d.x0, d.ihx, xsorted, px, i0, di, wx);
-        const double dx1 = px - L.x1, dx2 = L.x2 - px;
-        const double dxa = dx1 * dx1, dxb = dx2 * dx2;
         // Screening pass, branch-free: every candidate of the row is
         // estimated from the node pairs (i0, i0 + 1) of rows o0 / o1 (in
         // bounds: i0 + 1 <= nx - 1, or the zero padding column), so the LDS
@@ -538,8 +704,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
             pass = 0u;
             const float* zr = d.Zf + i0;
             const float2* zp = d.Zp + i0;
-            const float wxf = di ? (float)wx : 0.0f, dxaf = (float)dxa, dxbf = (float)dxb;
-            float Mf = (float)(M * (1.0 + 0x1p-22));   // >= M: an upper bound of the minimum
+            const float wxf = di ? (float)wx : 0.0f;
             // Columns past the row's last kept one (centre-out order) have
             // +inf (or NaN) estimates that neither lower Mf nor pass colok, so
             // with kSide the loop stops at a chunk of four columns once no
@@ -552,7 +717,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
                 float za, zb;
                 if (kPairs) {
                     // (z, dz/dx) pairs: one 8-byte LDS load per row of the cell
-                    const float2 a0 = zp[o0[t]], a1 = zp[o1[t]];
+                    const float2 a0 = zp[o0[t] & 0xffff], a1 = zp[(unsigned)o0[t] >> 16];
                     za = __builtin_fmaf(a0.y, wxf, a0.x);
                     zb = __builtin_fmaf(a1.y, wxf, a1.x);
                 } else {
@@ -565,8 +730,9 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
                 }
                 const float ze = __builtin_fmaf(zb - za, wyf[t], za);
                 const float e1 = ze - L.z1f, e2 = L.z2f - ze;
-                const float A = __builtin_fmaf(e1, e1, dxaf + sy1[t]);
-                const float B = __builtin_fmaf(e2, e2, dxbf + sy2[t]);
+                const float dy2 = dyL - dyf[t];
+                const float A = __builtin_fmaf(e1, e1, __builtin_fmaf(dyf[t], dyf[t], dxaf));
+                const float B = __builtin_fmaf(e2, e2, __builtin_fmaf(dy2, dy2, dxbf));
                 const float os = __builtin_fmaf(L.n2f, sqrt_f32(B), L.n1f * sqrt_f32(A));
                 // Mf = min(Mf, (est + D32)(1 + 3 kRho32)); NaN and +inf (dropped
                 // column) estimates leave it; the probe's estimate bounds its
