@@ -514,7 +514,7 @@ __device__ __forceinline__ float row_bound2d(float a1, float a2, float c1, float
 
 template <int G, bool kSplit, bool kPairs, bool kSide, bool kReuse>
 __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, double R, double& bestX, double& bestY,
-                                             double& minOpt)
+                                             double& minOpt, int lev)
 {
     constexpr int C = (G - 1) / 2;
     const double minX = bestX - R, maxX = bestX + R;
@@ -606,7 +606,9 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         if (kSide && kk > 0 && (cdone & side)) continue;
         const double pya = __builtin_fma((double)j, stepY, minY);
         const double a = fmax(fabs(pya - L.y1) - epsy, 0.0), b = fmax(fabs(pya - L.y2) - epsy, 0.0);
-        if (lower_bound32(a, b, g1f, g2f, L.n1f, L.n2f) > M) {
+        // (on the last level, whose 7.4 m-wide window keeps nearly every
+        // column, the bound is not evaluated: it cost more than it saved)
+        if (lev < 4 && lower_bound32(a, b, g1f, g2f, L.n1f, L.n2f) > M) {
             if (kSide && kk > 0) {
                 const double q = (kk & 1) ? pya + epsy : pya - epsy;
                 const float sl = slope32(q - L.y1, q - L.y2, g1f, g2f, L.n1f, L.n2f);
@@ -676,11 +678,12 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         const double dxa = dx1 * dx1, dxb = dx2 * dx2;
         const float dxaf = (float)dxa, dxbf = (float)dxb;
         float Mf = (float)(M * (1.0 + 0x1p-22));   // >= M: an upper bound of the minimum
-        // Row bound over both axes (11-wide levels, once the one-axis bound
-        // kept the row): the least path through the row's x at the window's
-        // height range, row_bound2d below.  A row it prunes needs nothing
-        // more, not even its x lookup.
-        if (G == 11 && L.screen) {
+        // Row bound over both axes (the last two levels, once the one-axis
+        // bound kept the row): the least path through the row's x at the
+        // window's height range, row_bound2d.  A row it prunes needs nothing
+        // more, not even its x lookup.  On the wider windows of the earlier
+        // levels it pruned almost no row the one-axis bound kept.
+        if (G == 11 && L.screen && lev >= 3) {
             // end points' y relative to the window's low edge (recomputed per
             // row rather than kept live through the row loop)
             const float wy1 = (float)(L.y1 - minY), wy2 = (float)(L.y2 - minY), wspan = (float)spanY;
@@ -785,11 +788,11 @@ __device__ __forceinline__ void fermat_solve(const DemDev& d, double x1, double 
     double R = std_max(__builtin_sqrt(ex * ex + ey * ey), 500.0);    // :483-484
     double minOpt = 1e9;                                             // :497
     const Leg L = make_leg(d, x1, y1, z1, x2, y2, z2, n1, n2);
-    fermat_level<15, kSplit, kPairs, kSide, kReuse>(d, L, R, bestX, bestY, minOpt);
+    fermat_level<15, kSplit, kPairs, kSide, kReuse>(d, L, R, bestX, bestY, minOpt, 0);
     R /= 5.0;                                                        // :551
 #pragma unroll 1
     for (int level = 1; level < 5; ++level) {
-        fermat_level<11, kSplit, kPairs, kSide, kReuse>(d, L, R, bestX, bestY, minOpt);
+        fermat_level<11, kSplit, kPairs, kSide, kReuse>(d, L, R, bestX, bestY, minOpt, level);
         R /= 3.0;
     }
     qx = bestX;                                                      // :555-557
