@@ -486,7 +486,7 @@ __device__ __forceinline__ float row_slack(float c1, float c2, float w, float n1
 }
 
 __device__ __forceinline__ float row_bound2d(float a1, float a2, float c1, float c2, float w, float n1, float n2,
-                                             float slack)
+                                             float slack, float& upper)
 {
     // start: the straight line between the end points, weighted by their
     // distances from the row
@@ -509,6 +509,7 @@ __device__ __forceinline__ float row_bound2d(float a1, float a2, float c1, float
     const float h = __builtin_fmaf(n1 * q1, s1, (n2 * q2) * s2);
     const float d = __builtin_fmaf(n1 * t1, s1, (n2 * t2) * s2);
     const float drop = __builtin_fminf(d * (0.0f - y), d * (w - y));
+    upper = __builtin_fmaf(h, 1.0f + 2e-5f, slack);   // >= h(y~) >= the row's least path
     return __builtin_fmaf(h, (float)kDefl, drop) - slack;
 }
 
@@ -648,6 +649,7 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
     const float Dk3 = (float)(L.D * (1.0 + 3.0 * kRho32)), D2 = (float)(2.0 * L.D);
 
     unsigned rdone = 0u;   // side stop of the rows (bit 0: + side, bit 1: - side)
+    float upP = __builtin_inff(), upM = __builtin_inff();   // row_bound2d's upper value, last row per side
 #pragma unroll 1
     for (int k = 0; k < G; ++k) {
         const int i = center_out(k, C);
@@ -687,9 +689,22 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
             // end points' y relative to the window's low edge (recomputed per
             // row rather than kept live through the row loop)
             const float wy1 = (float)(L.y1 - minY), wy2 = (float)(L.y2 - minY), wspan = (float)spanY;
-            if (row_bound2d(dxaf + g1f, dxbf + g2f, wy1, wy2, wspan, L.n1f, L.n2f,
-                            row_slack(wy1, wy2, wspan, L.n1f, L.n2f)) > Mf)
+            float up;
+            const float b = row_bound2d(dxaf + g1f, dxbf + g2f, wy1, wy2, wspan, L.n1f, L.n2f,
+                                        row_slack(wy1, wy2, wspan, L.n1f, L.n2f), up);
+            // Side stop: the row's least path m(x) is convex in x (a minimum
+            // over y of a jointly convex path), so when this pruned row's
+            // bound exceeds the upper value of the previous row on its side,
+            // m rises outward from here and every further row on that side is
+            // above Mf too (Mf only decreases).
+            const bool minus = (k & 1) != 0;
+            if (b > Mf) {
+                if (kSide && k > 0 && b > (minus ? upM : upP)) rdone |= minus ? 2u : 1u;
+                if (minus) upM = up; else upP = up;
                 continue;
+            }
+            if (k == 0) upM = up;
+            if (minus) upM = up; else upP = up;
         }
         int i0 = 0, di = 0;
         double wx = 0.0;
