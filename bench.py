@@ -203,7 +203,7 @@ def strided(a, m):
     return np.linspace(0, len(a) - 1, m).round().astype(int)
 
 
-def cpu_baseline(sc, root, side: int = 16) -> tuple[dict, dict | None]:
+def cpu_baseline(sc, root, side: int = 20) -> tuple[dict, dict | None]:
     """The reference's own OpenMP back-projection on a strided side^3 voxel
     subset of the same workload: oracle/_ref/proc_test_script_env, the
     reference's proc_test_script built from its sources with the radar
