@@ -89,7 +89,10 @@ __device__ __forceinline__ double r1_mean_fp64(const double* __restrict__ ax, co
 // Plain FP64 body, proc_test_script.cpp:484-668 in the reference's operation
 // order (built with -ffp-contract=off); the R1 means take faithful square
 // roots (r1_mean_fp64).  Both bodies run the search with the side stop when
-// the DEM is in LDS (kSplit); this one without the probe reuse (kReuse).
+// the DEM is in LDS (kSplit); this one without the probe reuse (kReuse) and
+// with the 15-wide first level in two column parts (the mixed body runs it in
+// one: with its packed column terms that measured 1.5 % faster, while this
+// body measured 3 % slower without the split).
 template <bool kSplit = false, bool kPairs = false>
 __device__ __forceinline__ void bp_fp64_thread(const DemDev& d, const BpArgs& a, int g, int64_t v, int p0, int p1)
 {
@@ -216,8 +219,8 @@ __device__ __forceinline__ void bp_mixed_thread(const DemDev& d, const BpArgs& a
 
     for (int i = p0; i < p1; ++i) {
         double qax, qay, qaz, qbx, qby, qbz;
-        fermat_solve<kSplit, kPairs, kSplit>(d, a.traj.tx[i], a.traj.ty[i], a.traj.tz[i], Px, Py, Pz, n1, n2, qax, qay, qaz);
-        fermat_solve<kSplit, kPairs, kSplit>(d, Px, Py, Pz, a.traj.rx[i], a.traj.ry[i], a.traj.rz[i], n2, n1, qbx, qby, qbz);
+        fermat_solve<false, kPairs, kSplit>(d, a.traj.tx[i], a.traj.ty[i], a.traj.tz[i], Px, Py, Pz, n1, n2, qax, qay, qaz);
+        fermat_solve<false, kPairs, kSplit>(d, Px, Py, Pz, a.traj.rx[i], a.traj.ry[i], a.traj.rz[i], n2, n1, qbx, qby, qbz);
         {
 #pragma clang fp contract(fast)
             double e0 = Px - qax, e1 = Py - qay, e2 = Pz - qaz;
