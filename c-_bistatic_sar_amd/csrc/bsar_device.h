@@ -608,8 +608,9 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         const double pya = __builtin_fma((double)j, stepY, minY);
         const double a = fmax(fabs(pya - L.y1) - epsy, 0.0), b = fmax(fabs(pya - L.y2) - epsy, 0.0);
         // (on the last level, whose 7.4 m-wide window keeps nearly every
-        // column, the bound is not evaluated: it cost more than it saved)
-        if (lev < 4 && lower_bound32(a, b, g1f, g2f, L.n1f, L.n2f) > M) {
+        // column, the back-projection kernels (kSide) skip the bound: it
+        // cost them more than it saved)
+        if ((!kSide || lev < 4) && lower_bound32(a, b, g1f, g2f, L.n1f, L.n2f) > M) {
             if (kSide && kk > 0) {
                 const double q = (kk & 1) ? pya + epsy : pya - epsy;
                 const float sl = slope32(q - L.y1, q - L.y2, g1f, g2f, L.n1f, L.n2f);
@@ -680,12 +681,14 @@ __device__ __forceinline__ void fermat_level(const DemDev& d, const Leg& L, doub
         const double dxa = dx1 * dx1, dxb = dx2 * dx2;
         const float dxaf = (float)dxa, dxbf = (float)dxb;
         float Mf = (float)(M * (1.0 + 0x1p-22));   // >= M: an upper bound of the minimum
-        // Row bound over both axes (the last two levels, once the one-axis
-        // bound kept the row): the least path through the row's x at the
-        // window's height range, row_bound2d.  A row it prunes needs nothing
-        // more, not even its x lookup.  On the wider windows of the earlier
-        // levels it pruned almost no row the one-axis bound kept.
-        if (G == 11 && L.screen && lev >= 3) {
+        // Row bound over both axes (the last two levels of the back-projection
+        // kernels' search, once the one-axis bound kept the row): the least
+        // path through the row's x at the window's height range, row_bound2d.
+        // A row it prunes needs nothing more, not even its x lookup.  On the
+        // wider windows of the earlier levels it pruned almost no row the
+        // one-axis bound kept, and the search-only kernels (3 waves per SIMD,
+        // no kSide) measured 3 % slower with it.
+        if (kSide && G == 11 && L.screen && lev >= 3) {
             // end points' y relative to the window's low edge (recomputed per
             // row rather than kept live through the row loop)
             const float wy1 = (float)(L.y1 - minY), wy2 = (float)(L.y2 - minY), wspan = (float)spanY;
