@@ -28,6 +28,18 @@ __device__ __forceinline__ double sqrt_faithful(double a)
 // Conditioning limit of the expanded squared distances in r1_mean_fp64.
 constexpr double kR1Cond = 32.0;
 
+// The antenna arrays are read-only for a whole launch: reading them through
+// the constant address space lets the wave-uniform loads of the R1 sums be
+// scalar loads also where the arrays' addresses come from memory (the
+// serving kernel's job table) rather than from kernel arguments.
+template <class T>
+using const_as_ptr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ __forceinline__ const_as_ptr<T> as_const(const T* p)
+{
+    return (const_as_ptr<T>)p;
+}
+
 // Mean distance from the n antennas (ax, ay, az) to q (proc:518-548),
 // summed in antenna order with faithful square roots.
 //
@@ -58,10 +70,10 @@ __device__ __forceinline__ double r1_mean_fp64(const double* __restrict__ ax, co
     const double Q = __builtin_fma(uz, uz, __builtin_fma(uy, uy, ux * ux));
     const double gap = fmax(fmax(g.zlo - qz, qz - g.zhi), 0.0);
     if (g.bsq + Q <= kR1Cond * (gap * gap)) {
-        const double* __restrict__ bx = gb;
-        const double* __restrict__ by = gb + g.stride;
-        const double* __restrict__ bz = gb + 2 * (int64_t)g.stride;
-        const double* __restrict__ bc = gb + 3 * (int64_t)g.stride;
+        const const_as_ptr<double> bx = as_const(gb);
+        const const_as_ptr<double> by = as_const(gb + g.stride);
+        const const_as_ptr<double> bz = as_const(gb + 2 * (int64_t)g.stride);
+        const const_as_ptr<double> bc = as_const(gb + 3 * (int64_t)g.stride);
         const double mx = -2.0 * ux, my = -2.0 * uy, mz = -2.0 * uz;
 #pragma unroll 4
         for (int j = 0; j < n; ++j) {
@@ -69,9 +81,10 @@ __device__ __forceinline__ double r1_mean_fp64(const double* __restrict__ ax, co
             s += sqrt_faithful(a);
         }
     } else {
+        const const_as_ptr<double> cx = as_const(ax), cy = as_const(ay), cz = as_const(az);
 #pragma unroll 4
         for (int j = 0; j < n; ++j) {
-            const double f0 = ax[j] - qx, f1 = ay[j] - qy, f2 = az[j] - qz;
+            const double f0 = cx[j] - qx, f1 = cy[j] - qy, f2 = cz[j] - qz;
             const double a = __builtin_fma(f2, f2, __builtin_fma(f1, f1, f0 * f0));
             s += sqrt_faithful(a);
         }
@@ -204,7 +217,7 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
 
 // Mixed-precision body: exact FP64 refraction points, FP64 delay and phase
 // argument, tabulated R1 means, FP32 interpolation and Kahan accumulation.
-template <bool kSplit = false, bool kPairs = false>
+template <bool kSplit = false, bool kPairs = false, bool kSplit15 = false>
 __device__ __forceinline__ void bp_mixed_thread(const DemDev& d, const BpArgs& a, const MixedArgs& m, int g,
                                                 int64_t v, int p0, int p1, unsigned& exact)
 {
@@ -219,8 +232,8 @@ __device__ __forceinline__ void bp_mixed_thread(const DemDev& d, const BpArgs& a
 
     for (int i = p0; i < p1; ++i) {
         double qax, qay, qaz, qbx, qby, qbz;
-        fermat_solve<false, kPairs, kSplit>(d, a.traj.tx[i], a.traj.ty[i], a.traj.tz[i], Px, Py, Pz, n1, n2, qax, qay, qaz);
-        fermat_solve<false, kPairs, kSplit>(d, Px, Py, Pz, a.traj.rx[i], a.traj.ry[i], a.traj.rz[i], n2, n1, qbx, qby, qbz);
+        fermat_solve<kSplit15, kPairs, kSplit>(d, a.traj.tx[i], a.traj.ty[i], a.traj.tz[i], Px, Py, Pz, n1, n2, qax, qay, qaz);
+        fermat_solve<kSplit15, kPairs, kSplit>(d, Px, Py, Pz, a.traj.rx[i], a.traj.ry[i], a.traj.rz[i], n2, n1, qbx, qby, qbz);
         {
 #pragma clang fp contract(fast)
             double e0 = Px - qax, e1 = Py - qay, e2 = Pz - qaz;

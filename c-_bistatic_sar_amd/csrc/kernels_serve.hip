@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256, kLds ? 4 : 2) void k_serve(const SrvJobDev* __
             const int64_t v = w.vox0 + threadIdx.x;
             if (kMixed) {
                 unsigned exact = 0;
-                bp_mixed_thread<kLds, kLds>(d, J.a, J.m, w.g, v, w.p0, w.p1, exact);
+                bp_mixed_thread<kLds, kLds, kLds>(d, J.a, J.m, w.g, v, w.p0, w.p1, exact);
                 if (exact && J.m.exact_count) atomicAdd(J.m.exact_count, (unsigned long long)exact);
             } else {
                 bp_fp64_thread<kLds, kLds>(d, J.a, w.g, v, w.p0, w.p1);
