@@ -167,7 +167,7 @@ def roofline(rep: dict, n: int, form: str, flops_prof: str = PROFILE_FLOPS, traf
     out["effective_tflops"] = round(eff / 1e12, 3)
     out["reference_flop_per_voxel_pulse"] = {"fp64": f64, "fp32": f32}
     out["peak_note"] = ("published vector peaks 78.6 (FP64) and 157.3 (FP32) TFLOP/s combined for the executed "
-                        "FP64/FP32 mix; non-packed FP32 issues at twice the FP64 rate (tools/valu_rates.hip)")
+                        "FP64/FP32 mix; non-packed FP32 issues at twice the FP64 rate (c-_bistatic_sar_amd/tools/valu_rates.hip)")
     return out
 
 
