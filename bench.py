@@ -14,9 +14,9 @@ range-compressed data already resident in HBM.  After the timed steps the
 default run adds: the plain FP64 image (and PSLR/ISLR of the mixed one
 against it), parity of both images against the oracle on a strided voxel
 subset, the reference's own OpenMP proc_test_script at the workload's radar
-constants on a 16^3 strided subset (the CPU baseline, whose image is
+constants on a 20^3 strided subset (the CPU baseline, whose image is
 compared with the GPU's), and the 256-job serving stream (mixed, plus FP64 on
-its first 16 jobs) with p50/p99 job latency.  Prints one JSON line on
+its first 32 jobs) with p50/p99 job latency.  Prints one JSON line on
 stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -609,7 +609,7 @@ def main() -> int:
     ap.add_argument("--check-solves", type=int, default=100000, help="rough: solves checked against the CPU")
     ap.add_argument("--no-serve", action="store_true", help="large/small: skip the serving stream")
     ap.add_argument("--no-parity", action="store_true", help="large/small: skip the oracle parity check")
-    ap.add_argument("--serve-fp64-jobs", type=int, default=16,
+    ap.add_argument("--serve-fp64-jobs", type=int, default=32,
                     help="large/small: FP64 serving on this prefix of the stream (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -482,6 +482,7 @@ int bsar_upsample(bsar_ctx* c, const double* in, int S, int N, int factor, doubl
     if (!c) return BSAR_ERR_INVALID;
     if (S < 2 || N < 0 || factor < 1 || (N && (!in || !out)))
         return set_error(c, BSAR_ERR_INVALID, "upsample needs S >= 2, N >= 0, factor >= 1");
+    if ((int64_t)S * factor >= (int64_t)1 << 30) return set_error(c, BSAR_ERR_INVALID, "S * factor too large");
     BSAR_HIP(c, hipSetDevice(c->device));
     if (N == 0) return BSAR_OK;
     const size_t nin = (size_t)S * N, nout = nin * factor;
@@ -489,7 +490,7 @@ int bsar_upsample(bsar_ctx* c, const double* in, int S, int N, int factor, doubl
     BSAR_HIP(c, hipMalloc(&bi.p, nin * sizeof(double2)));
     BSAR_HIP(c, hipMalloc(&bo.p, nout * sizeof(double2)));
     BSAR_HIP(c, hipMemcpyAsync(bi.p, in, nin * sizeof(double2), hipMemcpyHostToDevice, c->stream));
-    BSAR_HIP(c, launch_spline((const double2*)bi.p, S, N, factor, (double2*)bo.p, c->stream));
+    BSAR_HIP(c, launch_spline((const double2*)bi.p, S, N, factor, (double2*)bo.p, nullptr, c->stream));
     BSAR_HIP(c, hipMemcpyAsync(out, bo.p, nout * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
     BSAR_HIP(c, hipStreamSynchronize(c->stream));
     return BSAR_OK;
@@ -622,13 +623,16 @@ __global__ void k_to_f32(const double2* __restrict__ a, float2* __restrict__ b, 
     if (k < n) b[k] = make_float2((float)a[k].x, (float)a[k].y);
 }
 
+// FP32 copy of the upsampled samples for the mixed form (when the spline did
+// not write it in the same pass), then the dataset's byte count.
 static int finish_rcomp(bsar_ctx* c, bsar_dataset* ds)
 {
-    // FP32 copy of the upsampled samples for the mixed form.
     const int64_t n = (int64_t)ds->ndata * ds->S4;
-    BSAR_HIP(c, hipMalloc((void**)&ds->rc32, (size_t)n * sizeof(float2)));
-    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, ds->rc, ds->rc32, n);
-    BSAR_HIP(c, hipGetLastError());
+    if (!ds->rc32) {
+        BSAR_HIP(c, hipMalloc((void**)&ds->rc32, (size_t)n * sizeof(float2)));
+        hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, ds->rc, ds->rc32, n);
+        BSAR_HIP(c, hipGetLastError());
+    }
     BSAR_HIP(c, hipStreamSynchronize(c->stream));
     ds->bytes += n * (int64_t)(sizeof(double2) + sizeof(float2));
     return BSAR_OK;
@@ -657,7 +661,8 @@ int bsar_dataset_load_rootdata(bsar_ctx* c, bsar_dataset* ds, const double* root
     BSAR_HIP(c, hipMalloc(&bi.p, nin * sizeof(double2)));
     BSAR_HIP(c, hipMemcpyAsync(bi.p, root, nin * sizeof(double2), hipMemcpyHostToDevice, c->stream));
     BSAR_HIP(c, hipMalloc((void**)&ds->rc, nin * ds->p.upsample * sizeof(double2)));
-    BSAR_HIP(c, launch_spline((const double2*)bi.p, S, n_data, ds->p.upsample, ds->rc, c->stream));
+    BSAR_HIP(c, hipMalloc((void**)&ds->rc32, nin * ds->p.upsample * sizeof(float2)));
+    BSAR_HIP(c, launch_spline((const double2*)bi.p, S, n_data, ds->p.upsample, ds->rc, ds->rc32, c->stream));
     ds->ndata = n_data;
     ds->S4 = (int)S4;
     return finish_rcomp(c, ds);
@@ -683,7 +688,8 @@ int bsar_dataset_load_rootdata_device(bsar_ctx* c, bsar_dataset* ds, const void*
         src = (const double2*)wide.p;
     }
     BSAR_HIP(c, hipMalloc((void**)&ds->rc, nin * ds->p.upsample * sizeof(double2)));
-    BSAR_HIP(c, launch_spline(src, S, n_data, ds->p.upsample, ds->rc, c->stream));
+    BSAR_HIP(c, hipMalloc((void**)&ds->rc32, nin * ds->p.upsample * sizeof(float2)));
+    BSAR_HIP(c, launch_spline(src, S, n_data, ds->p.upsample, ds->rc, ds->rc32, c->stream));
     ds->ndata = n_data;
     ds->S4 = (int)S4;
     return finish_rcomp(c, ds);

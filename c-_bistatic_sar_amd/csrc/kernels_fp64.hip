@@ -17,6 +17,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace bsar {
 
 __global__ __launch_bounds__(256) void k_dem_interp(DemDev dem, const double* __restrict__ xy, int64_t m,
@@ -68,19 +70,21 @@ __global__ __launch_bounds__(dem_block(kMode, 3), dem_waves(kMode, 3)) void k_sl
     volta[o] = qx; volta[o + 1] = qy; volta[o + 2] = qz;
 }
 
-// splineInterpolation, interpolation.cpp:4-51: one thread per output sample.
-// in: [N][S] complex, out: [N][S*f] complex.
+// splineInterpolation, interpolation.cpp:4-51: one thread per output
+// sample i of a pulse (grid x), pulses along grid y (and a stride loop when
+// there are more than gridDim.y of them).  in: [N][S] complex, out: [N][S*f]
+// complex; out32, when not null, receives the same samples rounded to
+// complex64 in the same pass (the mixed form's copy).  The per-sample terms
+// (position, index, Hermite weights) depend on i alone and are computed once
+// per thread.
 __global__ __launch_bounds__(256) void k_spline(const double2* __restrict__ in, int S, int N, int f,
-                                                double2* __restrict__ out)
+                                                double2* __restrict__ out, float2* __restrict__ out32)
 {
-    const int64_t S4 = (int64_t)S * f;
-    int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= S4 * N) return;
-    const int64_t p = k / S4;
-    const int64_t i = k - p * S4;
-    const double2* d = in + p * S;
+    const int S4 = S * f;   // < 2^30 (launch_spline)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S4) return;
     double pos = (double)i / f;                                       // :18
-    int64_t idx = (int64_t)pos;                                       // :19
+    int idx = (int)pos;                                               // :19
     double frac = pos - (double)idx;                                  // :20
     if (idx >= S - 1) { idx = S - 2; frac = 1.0; }                    // :22-25
     const double t = frac, t2 = t * t, t3 = t2 * t;
@@ -88,14 +92,20 @@ __global__ __launch_bounds__(256) void k_spline(const double2* __restrict__ in, 
     const double h2 = -2 * t3 + 3 * t2;
     const double h3 = t3 - 2 * t2 + t;
     const double h4 = t3 - t2;
-    const double2 a = d[idx], b = d[idx + 1];
-    double2 dp = make_double2(0.0, 0.0), dn = make_double2(0.0, 0.0);
-    if (idx > 0) { double2 c = d[idx - 1]; dp = make_double2(a.x - c.x, a.y - c.y); }          // :39,41
-    if (idx < S - 2) { double2 c = d[idx + 2]; dn = make_double2(c.x - b.x, c.y - b.y); }      // :40,42
-    double2 r;
-    r.x = h1 * a.x + h2 * b.x + h3 * dp.x + h4 * dn.x;                // :45
-    r.y = h1 * a.y + h2 * b.y + h3 * dp.y + h4 * dn.y;                // :46
-    out[k] = r;
+    const bool hasp = idx > 0, hasn = idx < S - 2;
+    for (int p = blockIdx.y; p < N; p += gridDim.y) {
+        const double2* d = in + (int64_t)p * S;
+        const double2 a = d[idx], b = d[idx + 1];
+        double2 dp = make_double2(0.0, 0.0), dn = make_double2(0.0, 0.0);
+        if (hasp) { const double2 c = d[idx - 1]; dp = make_double2(a.x - c.x, a.y - c.y); }      // :39,41
+        if (hasn) { const double2 c = d[idx + 2]; dn = make_double2(c.x - b.x, c.y - b.y); }      // :40,42
+        double2 r;
+        r.x = h1 * a.x + h2 * b.x + h3 * dp.x + h4 * dn.x;            // :45
+        r.y = h1 * a.y + h2 * b.y + h3 * dp.y + h4 * dn.y;            // :46
+        const int64_t k = (int64_t)p * S4 + i;
+        out[k] = r;
+        if (out32) out32[k] = make_float2((float)r.x, (float)r.y);
+    }
 }
 
 // Plain FP64 back-projection.  Thread = (pulse group g, voxel v); it carries
@@ -167,11 +177,13 @@ hipError_t launch_slant_range(const DemDev& dem, const Traj& tr, const double* p
     return hipGetLastError();
 }
 
-hipError_t launch_spline(const double2* in, int S, int N, int f, double2* out, hipStream_t s)
+hipError_t launch_spline(const double2* in, int S, int N, int f, double2* out, float2* out32, hipStream_t s)
 {
-    const int64_t m = (int64_t)S * f * N;
-    if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_spline, dim3(blocks_for(m, 256)), dim3(256), 0, s, in, S, N, f, out);
+    const int64_t S4 = (int64_t)S * f;
+    if (S < 2 || f < 1 || S4 >= (int64_t)1 << 30) return hipErrorInvalidValue;
+    if (N <= 0) return hipSuccess;
+    const unsigned gy = (unsigned)std::min(N, 65535);
+    hipLaunchKernelGGL(k_spline, dim3(blocks_for(S4, 256), gy), dim3(256), 0, s, in, S, N, f, out, out32);
     return hipGetLastError();
 }
 

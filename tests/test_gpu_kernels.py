@@ -116,6 +116,38 @@ def test_upsample_batch_vs_oracle(ctx, oracle):
     assert bits_equal(ctx.upsample(root, 4).view(np.float64), oracle.upsample(root, 4).view(np.float64))
 
 
+def test_upsample_more_pulses_than_grid_rows(ctx, oracle):
+    # the kernel's grid holds at most 65535 pulses along y and loops over the rest
+    rng = np.random.default_rng(6)
+    root = rng.normal(size=(70001, 3)) + 1j * rng.normal(size=(70001, 3))
+    assert bits_equal(ctx.upsample(root, 3).view(np.float64), oracle.upsample(root, 3).view(np.float64))
+
+
+def test_upsample_rejects_oversized_columns(bsar, ctx):
+    # S * factor >= 2^30 is refused before any work (no pulses: nothing to allocate)
+    with pytest.raises(bsar.BsarError):
+        ctx.upsample(np.zeros((0, 1 << 15), np.complex128), 1 << 15)
+
+
+def test_fused_fp32_copy_matches_conversion(bsar, ctx, oracle):
+    # load_rootdata writes the complex64 copy in the spline's pass;
+    # load_upsampled converts the complex128 samples in a pass of its own:
+    # the mixed images (which read only the complex64 copy) are bit-identical
+    from _scenes import Tiny
+    t = Tiny(n=12, S=256, grid=(5, 4, 3))
+    root = t.random_root()
+    imgs = []
+    for fused in (True, False):
+        ds = ctx.dataset(t.params(bsar), bsar.Dem(*t.dem), t.tx, t.rx)
+        if fused:
+            ds.load_rootdata(root)
+        else:
+            ds.load_upsampled(oracle.upsample(root, t.up))
+        imgs.append(ds.backproject(t.gx, t.gy, t.gz, bsar.MIXED)[0])
+        ds.close()
+    assert bits_equal(imgs[0].view(np.float64), imgs[1].view(np.float64))
+
+
 @pytest.mark.parametrize("S", [2, 8, 64, 1024, 8192, 16384, 32768])   # LDS, split (16384) and global (32768) paths
 @pytest.mark.parametrize("conj", [True, False])
 def test_range_compress_fp64(ctx, oracle, S, conj):
