@@ -680,16 +680,13 @@ int bsar_dataset_load_rootdata_device(bsar_ctx* c, bsar_dataset* ds, const void*
     BSAR_HIP(c, hipSetDevice(c->device));
     if (ds->rc) drop_rcomp(ds);
     const size_t nin = (size_t)S * n_data;
-    DevBuf wide;
-    const double2* src = (const double2*)d_root;
-    if (precision == BSAR_MIXED) {
-        BSAR_HIP(c, hipMalloc(&wide.p, nin * sizeof(double2)));
-        BSAR_HIP(c, to_f64((const float2*)d_root, (double2*)wide.p, (int64_t)nin, c->stream));
-        src = (const double2*)wide.p;
-    }
     BSAR_HIP(c, hipMalloc((void**)&ds->rc, nin * ds->p.upsample * sizeof(double2)));
     BSAR_HIP(c, hipMalloc((void**)&ds->rc32, nin * ds->p.upsample * sizeof(float2)));
-    BSAR_HIP(c, launch_spline(src, S, n_data, ds->p.upsample, ds->rc, ds->rc32, c->stream));
+    // complex64 samples (BSAR_MIXED) are widened exactly inside the spline
+    if (precision == BSAR_MIXED)
+        BSAR_HIP(c, launch_spline((const float2*)d_root, S, n_data, ds->p.upsample, ds->rc, ds->rc32, c->stream));
+    else
+        BSAR_HIP(c, launch_spline((const double2*)d_root, S, n_data, ds->p.upsample, ds->rc, ds->rc32, c->stream));
     ds->ndata = n_data;
     ds->S4 = (int)S4;
     return finish_rcomp(c, ds);

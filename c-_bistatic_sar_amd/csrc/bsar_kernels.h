@@ -128,7 +128,9 @@ hipError_t launch_refraction(const DemDev& dem, const double* legs, int64_t m, d
 hipError_t launch_slant_range(const DemDev& dem, const Traj& tr, const double* pts, int64_t V, double n1,
                               double n2, double* ida, double* volta, hipStream_t s);
 // out32 (may be null): the output rounded to complex64 as well; S * f < 2^30.
+// complex64 input is widened exactly to complex128 on load.
 hipError_t launch_spline(const double2* in, int S, int N, int f, double2* out, float2* out32, hipStream_t s);
+hipError_t launch_spline(const float2* in, int S, int N, int f, double2* out, float2* out32, hipStream_t s);
 hipError_t launch_bp_fp64(const BpArgs& a, hipStream_t s);
 hipError_t launch_group_sum(const double2* acc, int groups, int64_t stride, int64_t nvox, double2* out,
                             hipStream_t s);

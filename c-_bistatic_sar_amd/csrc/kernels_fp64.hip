@@ -72,12 +72,17 @@ __global__ __launch_bounds__(dem_block(kMode, 3), dem_waves(kMode, 3)) void k_sl
 
 // splineInterpolation, interpolation.cpp:4-51: one thread per output
 // sample i of a pulse (grid x), pulses along grid y (and a stride loop when
-// there are more than gridDim.y of them).  in: [N][S] complex, out: [N][S*f]
-// complex; out32, when not null, receives the same samples rounded to
+// there are more than gridDim.y of them).  in: [N][S] complex (complex128,
+// or complex64 widened exactly to complex128 on load), out: [N][S*f]
+// complex128; out32, when not null, receives the same samples rounded to
 // complex64 in the same pass (the mixed form's copy).  The per-sample terms
 // (position, index, Hermite weights) depend on i alone and are computed once
 // per thread.
-__global__ __launch_bounds__(256) void k_spline(const double2* __restrict__ in, int S, int N, int f,
+__device__ __forceinline__ double2 widen(double2 a) { return a; }
+__device__ __forceinline__ double2 widen(float2 a) { return make_double2((double)a.x, (double)a.y); }
+
+template <typename In>
+__global__ __launch_bounds__(256) void k_spline(const In* __restrict__ in, int S, int N, int f,
                                                 double2* __restrict__ out, float2* __restrict__ out32)
 {
     const int S4 = S * f;   // < 2^30 (launch_spline)
@@ -94,11 +99,11 @@ __global__ __launch_bounds__(256) void k_spline(const double2* __restrict__ in, 
     const double h4 = t3 - t2;
     const bool hasp = idx > 0, hasn = idx < S - 2;
     for (int p = blockIdx.y; p < N; p += gridDim.y) {
-        const double2* d = in + (int64_t)p * S;
-        const double2 a = d[idx], b = d[idx + 1];
+        const In* d = in + (int64_t)p * S;
+        const double2 a = widen(d[idx]), b = widen(d[idx + 1]);
         double2 dp = make_double2(0.0, 0.0), dn = make_double2(0.0, 0.0);
-        if (hasp) { const double2 c = d[idx - 1]; dp = make_double2(a.x - c.x, a.y - c.y); }      // :39,41
-        if (hasn) { const double2 c = d[idx + 2]; dn = make_double2(c.x - b.x, c.y - b.y); }      // :40,42
+        if (hasp) { const double2 c = widen(d[idx - 1]); dp = make_double2(a.x - c.x, a.y - c.y); }   // :39,41
+        if (hasn) { const double2 c = widen(d[idx + 2]); dn = make_double2(c.x - b.x, c.y - b.y); }   // :40,42
         double2 r;
         r.x = h1 * a.x + h2 * b.x + h3 * dp.x + h4 * dn.x;            // :45
         r.y = h1 * a.y + h2 * b.y + h3 * dp.y + h4 * dn.y;            // :46
@@ -177,14 +182,25 @@ hipError_t launch_slant_range(const DemDev& dem, const Traj& tr, const double* p
     return hipGetLastError();
 }
 
-hipError_t launch_spline(const double2* in, int S, int N, int f, double2* out, float2* out32, hipStream_t s)
+template <typename In>
+static hipError_t launch_spline_t(const In* in, int S, int N, int f, double2* out, float2* out32, hipStream_t s)
 {
     const int64_t S4 = (int64_t)S * f;
     if (S < 2 || f < 1 || S4 >= (int64_t)1 << 30) return hipErrorInvalidValue;
     if (N <= 0) return hipSuccess;
     const unsigned gy = (unsigned)std::min(N, 65535);
-    hipLaunchKernelGGL(k_spline, dim3(blocks_for(S4, 256), gy), dim3(256), 0, s, in, S, N, f, out, out32);
+    hipLaunchKernelGGL(k_spline<In>, dim3(blocks_for(S4, 256), gy), dim3(256), 0, s, in, S, N, f, out, out32);
     return hipGetLastError();
+}
+
+hipError_t launch_spline(const double2* in, int S, int N, int f, double2* out, float2* out32, hipStream_t s)
+{
+    return launch_spline_t(in, S, N, f, out, out32, s);
+}
+
+hipError_t launch_spline(const float2* in, int S, int N, int f, double2* out, float2* out32, hipStream_t s)
+{
+    return launch_spline_t(in, S, N, f, out, out32, s);
 }
 
 hipError_t launch_bp_fp64(const BpArgs& a, hipStream_t s)

@@ -207,8 +207,8 @@ BSAR_API void bsar_dataset_destroy(bsar_dataset* ds);
 BSAR_API int bsar_dataset_load_rootdata(bsar_ctx* ctx, bsar_dataset* ds, const double* root, int S, int n_data);
 
 /* Same, from a caller-owned device buffer: d_root is [n_data][S] complex128
- * (BSAR_FP64) or complex64 (BSAR_MIXED, widened to FP64 before the spline),
- * e.g. the output of bsar_range_compress_device. */
+ * (BSAR_FP64) or complex64 (BSAR_MIXED, widened exactly to FP64 as the
+ * spline reads it), e.g. the output of bsar_range_compress_device. */
 BSAR_API int bsar_dataset_load_rootdata_device(bsar_ctx* ctx, bsar_dataset* ds, const void* d_root, int S, int n_data,
                                       int precision);
 
