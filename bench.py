@@ -469,6 +469,26 @@ def run_rough(args, pkg, scene) -> dict:
         d2 = np.sqrt(((q - legs[:, 3:6]) ** 2).sum(1))
         return (na * d1 + nb * d2) / sc.c
 
+    # The same sample through the unscreened wavefront reduction (one wave per
+    # leg, lanes over all 709 candidates), each leg set tiled to >= 5e5 legs:
+    # its rate beside the screened search's, and its points against the
+    # screened ones.
+    with pkg.Context(0) as ctx:
+        dem = pkg.Dem(*sc.dem)
+        wave_s, wave_n, wave_same = 0.0, 0, 0
+        for legs, got, na, nb in ((la, ga, sc.n1, sc.n2), (lb, gb, sc.n2, sc.n1)):
+            reps = max(1, -(-500_000 // len(legs)))
+            q, secs = ctx.refraction_points_wave(dem, np.tile(legs, (reps, 1)), na, nb)
+            wave_s += secs
+            wave_n += reps * len(legs)
+            wave_same += int((q[:len(legs)] == got).all(1).sum())
+    wave = {"solves": wave_n, "value": round(wave_n / wave_s, 1), "unit": "solve/s", "seconds": round(wave_s, 4),
+            "screened_over_wave": round(value / (wave_n / wave_s), 2), "identical_to_screened": wave_same,
+            "compared": n_check, "kernel": "k_refraction_wave",
+            "note": "one wavefront per leg, lanes over every candidate of each level, argmin with the lowest "
+                    "reference index among ties; the argmin-check sample tiled"}
+    log(f"wavefront reduction: {wave['value']:.3e} solves/s on {wave_n} solves; screened search "
+        f"{wave['screened_over_wave']}x faster; {wave_same}/{n_check} identical to the screened points")
     same = int((ga == ra).all(1).sum() + (gb == rb).all(1).sum())
     dtau = max(float(np.abs(delay(la, ga, sc.n1, sc.n2) - delay(la, ra, sc.n1, sc.n2)).max()),
                float(np.abs(delay(lb, gb, sc.n2, sc.n1) - delay(lb, rb, sc.n2, sc.n1)).max()))
@@ -498,6 +518,7 @@ def run_rough(args, pkg, scene) -> dict:
                              "commit": rcommit},
                          "effective_tflops": round(ach / 1e12, 3), "reference_flop_per_solve": FLOP_PER_SOLVE},
             "argmin_check": {"solves": n_check, "identical": same, "max_abs_dtau_s": dtau, "cpu": kind},
+            "wave_reduction": wave,
             "cpu_baseline": {"value": round(n_check / cpu_s, 1), "unit": "solve/s", "cores": threads, "kind": kind,
                              "sample": f"{n_check} solves (the argmin-check sample: {len(legs_a)} voxels x "
                                        f"{N} pulses x 2 legs), calculateRefractionPoint over {threads} threads"},

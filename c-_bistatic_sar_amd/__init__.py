@@ -97,6 +97,8 @@ SIGNATURES = {
     "bsar_ctx_synchronize": (C.c_int, [C.c_void_p]),
     "bsar_dem_interp": (C.c_int, [C.c_void_p, C.POINTER(DemC), _dp, C.c_int64, _dp]),
     "bsar_refraction_points": (C.c_int, [C.c_void_p, C.POINTER(DemC), _dp, C.c_int64, C.c_double, C.c_double, _dp]),
+    "bsar_refraction_points_wave": (C.c_int, [C.c_void_p, C.POINTER(DemC), _dp, C.c_int64, C.c_double, C.c_double,
+                                              _dp, _dp]),
     "bsar_slant_range_fermat": (C.c_int, [C.c_void_p, C.c_void_p, _dp, C.c_int64, _dp, _dp]),
     "bsar_slant_range_fermat_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                                  C.c_void_p, C.POINTER(ReportC)]),
@@ -267,6 +269,16 @@ class Context:
         self._chk(self.L.bsar_refraction_points(self.h, C.byref(d), _ptr(legs), len(legs), n1, n2, _ptr(out)),
                   "refraction_points")
         return out
+
+    def refraction_points_wave(self, dem: Dem, legs, n1: float, n2: float) -> tuple[np.ndarray, float]:
+        """refraction_points by the unscreened wavefront reduction; (points, kernel seconds)."""
+        legs = _f64(legs).reshape(-1, 6)
+        out = np.empty((len(legs), 3))
+        secs = C.c_double(0.0)
+        d = dem.c()
+        self._chk(self.L.bsar_refraction_points_wave(self.h, C.byref(d), _ptr(legs), len(legs), n1, n2, _ptr(out),
+                                                     C.byref(secs)), "refraction_points_wave")
+        return out, secs.value
 
     def upsample(self, data, factor: int = 4) -> np.ndarray:
         """data [N][S] complex -> [N][S*factor] complex."""

@@ -430,6 +430,33 @@ int bsar_refraction_points(bsar_ctx* c, const bsar_dem* dem, const double* legs,
     return BSAR_OK;
 }
 
+int bsar_refraction_points_wave(bsar_ctx* c, const bsar_dem* dem, const double* legs, int64_t m, double n1,
+                                double n2, double* out, double* seconds)
+{
+    if (!c) return BSAR_ERR_INVALID;
+    if (seconds) *seconds = 0.0;
+    if (m < 0 || m >= (int64_t)1 << 33 || (m && (!legs || !out))) return set_error(c, BSAR_ERR_INVALID, "bad leg buffer");
+    BSAR_HIP(c, hipSetDevice(c->device));
+    DemDev d;
+    int st = ctx_dem(c, dem, &d);
+    if (st) return st;
+    if (m == 0) return BSAR_OK;
+    if ((st = ctx_buffer(c, &c->in_buf, &c->in_cap, (size_t)m * 6 * sizeof(double)))) return st;
+    if ((st = ctx_buffer(c, &c->out_buf, &c->out_cap, (size_t)m * 3 * sizeof(double)))) return st;
+    BSAR_HIP(c, hipMemcpyAsync(c->in_buf, legs, (size_t)m * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    BSAR_HIP(c, hipEventRecord(c->ev0, c->stream));
+    BSAR_HIP(c, launch_refraction_wave(d, (const double*)c->in_buf, m, n1, n2, (double*)c->out_buf, c->stream));
+    BSAR_HIP(c, hipEventRecord(c->ev1, c->stream));
+    BSAR_HIP(c, hipMemcpyAsync(out, c->out_buf, (size_t)m * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BSAR_HIP(c, hipStreamSynchronize(c->stream));
+    if (seconds) {
+        float ms = 0.0f;
+        BSAR_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *seconds = ms * 1e-3;
+    }
+    return BSAR_OK;
+}
+
 int bsar_slant_range_fermat(bsar_ctx* c, const bsar_dataset* ds, const double* pts, int64_t V, double* ida,
                             double* volta)
 {

@@ -125,6 +125,9 @@ struct MixedArgs {
 hipError_t launch_dem_interp(const DemDev& dem, const double* xy, int64_t m, double* z, hipStream_t s);
 hipError_t launch_refraction(const DemDev& dem, const double* legs, int64_t m, double n1, double n2,
                              double* out, hipStream_t s);
+// The exhaustive form: one wave per leg, lanes over the candidates.
+hipError_t launch_refraction_wave(const DemDev& dem, const double* legs, int64_t m, double n1, double n2,
+                                  double* out, hipStream_t s);
 hipError_t launch_slant_range(const DemDev& dem, const Traj& tr, const double* pts, int64_t V, double n1,
                               double n2, double* ida, double* volta, hipStream_t s);
 // out32 (may be null): the output rounded to complex64 as well; S * f < 2^30.

@@ -2,6 +2,7 @@
 //
 //   k_dem_interp      interpolateDEMElevation, batched          (raw_data_processing.cpp:615-680)
 //   k_refraction      calculateRefractionPoint, batched legs    (raw_data_processing.cpp:455-572)
+//   k_refraction_wave the same search as a wavefront reduction over all 709 candidates
 //   k_slant_range     refraction points of calculaSlantRangeFermat (raw_data_processing.cpp:153-193)
 //   k_spline          splineInterpolation x factor per pulse    (interpolation.cpp:4-51)
 //   k_bp_fp64         fused plain-FP64 back-projection          (proc_test_script.cpp:484-668)
@@ -43,6 +44,86 @@ __global__ __launch_bounds__(dem_block(kMode, 3), dem_waves(kMode, 3)) void k_re
     out[3 * k] = qx;
     out[3 * k + 1] = qy;
     out[3 * k + 2] = qz;
+}
+
+// calculateRefractionPoint (raw_data_processing.cpp:455-572) as a wavefront-
+// parallel reduction over the candidates: one wave per leg, its 64 lanes
+// evaluate a level's G*G candidates exactly (lane l takes k = l, l + 64, ...
+// in increasing order, so its own first minimum is the sequential scan's),
+// then a butterfly argmin over the wave keeps the least value and, among
+// equal values, the lowest index k = i*G + j -- the candidate the reference's
+// strict '<' scan settles on.  The level's winner replaces the best point only
+// when it is strictly below the minimum of the previous levels (:536-541).
+// Every one of the 709 candidates is evaluated (no screening), in the same
+// expression as the screened search's exact evaluations.
+__device__ __forceinline__ void fermat_wave(const DemDev& d, double x1, double y1, double z1, double x2, double y2,
+                                            double z2, double n1, double n2, double& qx, double& qy, double& qz)
+{
+    double bestX = (x1 + x2) / 2.0;                                  // :478-479
+    double bestY = (y1 + y2) / 2.0;
+    const double ex = x2 - x1, ey = y2 - y1;
+    double R = std_max(__builtin_sqrt(ex * ex + ey * ey), 500.0);    // :483-484
+    double minOpt = 1e9;                                             // :497
+    const Leg L = make_leg(d, x1, y1, z1, x2, y2, z2, n1, n2);
+    const bool empty = (d.flags & DEM_EMPTY) != 0;
+    const bool xsorted = (d.flags & DEM_X_SORTED) != 0;
+    const bool ysorted = (d.flags & DEM_Y_SORTED) != 0;
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll 1
+    for (int level = 0; level < 5; ++level) {
+        const int G = level == 0 ? 15 : 11;                          // :508
+        const double minX = bestX - R, maxX = bestX + R;             // :509-512
+        const double minY = bestY - R, maxY = bestY + R;
+        const double spanX = maxX - minX, spanY = maxY - minY;
+        double v = __builtin_inf();
+        int kb = 0x7fffffff;
+        for (int k = lane; k < G * G; k += 64) {
+            const int i = k / G, j = k - (k / G) * G;
+            const double px = minX + i * spanX / (G - 1);            // :525-526
+            const double py = minY + j * spanY / (G - 1);
+            int i0 = 0, di = 0;
+            double wx = 0.0;
+            if (!empty) axis_lookup(d.X, d.nx, d.x0, d.ihx, xsorted, px, i0, di, wx);
+            const double dx1 = px - L.x1, dx2 = L.x2 - px;
+            const double o = exact_opt(d, L, empty, ysorted, i0, di, wx, dx1 * dx1, dx2 * dx2, py);   // :529-534
+            if (o < v) { v = o; kb = k; }                            // NaN never wins
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(v, off);
+            const int ok = __shfl_xor(kb, off);
+            if (ov < v || (ov == v && ok < kb)) { v = ov; kb = ok; }
+        }
+        if (v < minOpt) {                                            // :536-541
+            minOpt = v;
+            const int bi = kb / G, bj = kb - (kb / G) * G;
+            bestX = minX + bi * spanX / (G - 1);
+            bestY = minY + bj * spanY / (G - 1);
+        }
+        R /= (level == 0) ? 5.0 : 3.0;                               // :551
+    }
+    qx = bestX;                                                      // :555-557
+    qy = bestY;
+    qz = dem_interp(d, bestX, bestY);
+}
+
+// One wave per leg (legs [m][6] -> out [m][3]), fermat_wave.
+template <int kMode>
+__global__ __launch_bounds__(256) void k_refraction_wave(DemDev dem, const double* __restrict__ legs, int64_t m,
+                                                         double n1, double n2, double* __restrict__ out)
+{
+    extern __shared__ double lds[];
+    const DemDev d = dem_stage<kMode>(dem, lds);
+    const int64_t k = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;   // wave-uniform
+    if (k >= m) return;
+    const double* s = legs + 6 * k;
+    double qx, qy, qz;
+    fermat_wave(d, s[0], s[1], s[2], s[3], s[4], s[5], n1, n2, qx, qy, qz);
+    if ((threadIdx.x & 63) == 0) {
+        out[3 * k] = qx;
+        out[3 * k + 1] = qy;
+        out[3 * k + 2] = qz;
+    }
 }
 
 // Thread per (point v, pulse i): both legs of one calculaSlantRangeFermat
@@ -169,6 +250,28 @@ hipError_t launch_refraction(const DemDev& dem, const double* legs, int64_t m, d
     if (m <= 0) return hipSuccess;
     const DemPlan p = plan_dem(dem);
     BSAR_LAUNCH_DEM(k_refraction, 3, p, m, s, dem, legs, m, n1, n2, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_refraction_wave(const DemDev& dem, const double* legs, int64_t m, double n1, double n2,
+                                  double* out, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    const DemPlan p = plan_dem(dem);
+    const int64_t threads = m * 64;
+    // every mode stages at most what the screened kernels stage; blocks of 256
+    // threads hold four legs
+    if (p.mode == DEM_LDS)
+        hipLaunchKernelGGL(k_refraction_wave<DEM_LDS>, dim3(blocks_for(threads, 256)), dim3(256), p.bytes, s, dem,
+                           legs, m, n1, n2, out);
+    else if (p.mode == DEM_SPLIT) {
+        (void)hipFuncSetAttribute((const void*)k_refraction_wave<DEM_SPLIT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.bytes);
+        hipLaunchKernelGGL(k_refraction_wave<DEM_SPLIT>, dim3(blocks_for(threads, 256)), dim3(256), p.bytes, s, dem,
+                           legs, m, n1, n2, out);
+    } else
+        hipLaunchKernelGGL(k_refraction_wave<DEM_GLOBAL>, dim3(blocks_for(threads, 256)), dim3(256), 0, s, dem, legs,
+                           m, n1, n2, out);
     return hipGetLastError();
 }
 

@@ -113,6 +113,14 @@ BSAR_API int bsar_dem_interp(bsar_ctx* ctx, const bsar_dem* dem, const double* x
 BSAR_API int bsar_refraction_points(bsar_ctx* ctx, const bsar_dem* dem, const double* legs, int64_t m,
                            double n1, double n2, double* out);
 
+/* The same refraction points (calculateRefractionPoint, raw_data_processing.cpp:
+ * 455-572), computed without screening: one wavefront per leg, its lanes
+ * over every candidate of each level, reduced to the argmin with the lowest
+ * reference index among ties.  Bit-identical to bsar_refraction_points;
+ * *seconds (may be null) receives the kernel's device time.  m < 2^33. */
+BSAR_API int bsar_refraction_points_wave(bsar_ctx* ctx, const bsar_dem* dem, const double* legs, int64_t m,
+                                         double n1, double n2, double* out, double* seconds);
+
 /* Refraction points of calculaSlantRangeFermat, raw_data_processing.cpp:112-
  * 255, for V targets and the dataset's N pulses:
  * ida[v][i] = refraction(Tx_i -> P_v, n1, n2), volta[v][i] = refraction(P_v -> Rx_i, n2, n1).
