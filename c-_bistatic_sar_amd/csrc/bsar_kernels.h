@@ -75,8 +75,10 @@ struct R1Geo {
     double zlo, zhi;
 };
 
-// One back-projection launch: voxels [vox0, vox0 + nvox) of an nx*ny*nz
-// grid (x fastest), pulse groups g = 0..groups-1 each covering pulses
+// One back-projection launch: thread positions [vox0, vox0 + nvox) of an
+// nx*ny*nz grid (band_voxel maps a position to its voxel; accumulators and
+// images are indexed by voxel, x fastest), pulse groups g = 0..groups-1 each
+// covering pulses
 // [g*glen, min((g+1)*glen, n)), of which this launch adds the window
 // [woff, woff + wlen) to the group's running sums acc[g][v].
 struct BpArgs {

@@ -14,8 +14,8 @@ struct SrvJobDev {
     MixedArgs m;     // mixed form accumulates in m.acc
 };
 
-// Work item: voxels [vox0, vox0 + nvox) (nvox <= 256) of job `slot`, pulses
-// [p0, p1) of its pulse group g.
+// Work item: thread positions [vox0, vox0 + nvox) (nvox <= 256, mapped to
+// voxels by band_voxel) of job `slot`, pulses [p0, p1) of its pulse group g.
 struct SrvItem {
     int slot, g;
     int64_t vox0;

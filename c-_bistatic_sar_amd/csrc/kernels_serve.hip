@@ -1,10 +1,10 @@
 // kernels_serve.hip -- persistent back-projection kernel of the serving loop.
 //
 // A scheduling round hands the device one work list.  Every item is (job
-// slot, pulse group, 256-voxel tile, pulse window [p0, p1)); the resident
-// workgroups of each kernel variant pull that variant's items with an atomic
-// counter until they are exhausted, so tiles of different jobs run side by
-// side.  The per-voxel body is the one of the single-image kernels
+// slot, pulse group, tile of 256 thread positions (band_voxel), pulse window
+// [p0, p1)); the resident workgroups of each kernel variant pull that
+// variant's items with an atomic counter until they are exhausted, so tiles
+// of different jobs run side by side.  The per-voxel body is the one of the single-image kernels
 // (bsar_bp.h): each voxel adds its pulses in increasing order with the job's
 // own group split, so a job's image does not depend on what else shares the
 // device.  Every workgroup exits once the counter passes the variant's end.
