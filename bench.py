@@ -630,6 +630,10 @@ def main() -> int:
     ap.add_argument("--check-solves", type=int, default=100000, help="rough: solves checked against the CPU")
     ap.add_argument("--no-serve", action="store_true", help="large/small: skip the serving stream")
     ap.add_argument("--no-parity", action="store_true", help="large/small: skip the oracle parity check")
+    ap.add_argument("--traffic-calibration", action="store_true",
+                    help="large/small: also load the upsampled samples through bsar_dataset_load_upsampled, whose "
+                         "complex64 conversion (k_to_f32: reads 16 B, writes 8 B per sample) calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     ap.add_argument("--serve-fp64-jobs", type=int, default=32,
                     help="large/small: FP64 serving on this prefix of the stream (0: none)")
     args = ap.parse_args()
@@ -658,6 +662,15 @@ def main() -> int:
         log(f"forward model (GPU Fermat + FFT) {time.time() - t0:.1f} s")
         ds = ctx.dataset(pkg.Params(sc.fc, sc.c, sc.fs, sc.n1, sc.n2, sc.upsample), pkg.Dem(*sc.dem), sc.tx, sc.rx)
         ds.load_rootdata(root)
+        if args.traffic_calibration:
+            cal = ctx.dataset(pkg.Params(sc.fc, sc.c, sc.fs, sc.n1, sc.n2, sc.upsample), pkg.Dem(*sc.dem), sc.tx,
+                              sc.rx)
+            up = ctx.upsample(root, sc.upsample)
+            cal.load_upsampled(up)
+            log(f"traffic calibration: k_to_f32 over {up.size} complex128 samples ({up.size * 16} B read, "
+                f"{up.size * 8} B written)")
+            del up
+            cal.close()
         for w in range(args.warmup):
             _, rep = ds.backproject(sc.gx, sc.gy, sc.gz, pkg.MIXED)
             log(f"warmup {w + 1}/{args.warmup}: {rep['seconds']:.2f} s")
