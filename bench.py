@@ -402,11 +402,31 @@ def run_serve(args, pkg, scene) -> dict:
 FLOP_PER_SOLVE = 709 * 46 + 64     # one refraction search: 709 candidates x 46 flop + set-up
 
 
+def band_order(p: np.ndarray, nx: int, ny: int, nz: int, by: int = 8, bz: int = 8):
+    """Grid coordinates of positions p in the back-projection kernels' voxel
+    order: bands of `by` rows x `bz` planes, x-major inside a band, the band's
+    (y, z) cells innermost -- so the 64 points of a wavefront are one x column
+    of an 8 x 8 (y, z) block, which keeps the lanes' refraction searches on the
+    same branches."""
+    plane = nx * ny
+    c = p // (bz * plane)
+    bd = np.minimum(bz, nz - bz * c)
+    r = p - c * bz * plane
+    b = r // (by * bd * nx)
+    bh = np.minimum(by, ny - by * b)
+    w = r - b * by * bd * nx
+    cell = bh * bd
+    xi = w // cell
+    s = w - xi * cell
+    zz = s // bh
+    return xi, by * b + (s - zz * bh), bz * c + zz
+
+
 def run_rough(args, pkg, scene) -> dict:
     """Refraction-dominated workload: every refraction solve of a 128^3 grid
     buried 3-7 m under rough fBm terrain (167x100 DEM) for 1024 pulses, both
     legs (2 V N = 4.29e9 solves) through the device-buffer slant-range entry
-    point, in voxel chunks; then the argmin of a seeded sample of >= 1e5
+    point, in voxel chunks of the back-projection kernels' band order; then the argmin of a seeded sample of >= 1e5
     solves against the reference's own calculateRefractionPoint on the host
     (identical point, delay difference <= 1e-12 s)."""
     sc = scene.make_scene("rough")
@@ -427,7 +447,8 @@ def run_rough(args, pkg, scene) -> dict:
         for c0 in range(0, V, chunk):
             m = min(chunk, V - c0)
             v = np.arange(c0, c0 + m)
-            pts = np.stack([sc.gx[v % nx], sc.gy[(v // nx) % ny], sc.gz[v // (nx * ny)]], 1)
+            xi, yi, zi = band_order(v, nx, ny, len(sc.gz))
+            pts = np.stack([sc.gx[xi], sc.gy[yi], sc.gz[zi]], 1)
             bp.write(pts)
             rep = ds.slant_range_fermat_device(bp.ptr, m, ba.ptr, bb.ptr)
             dev_s += rep["seconds"]

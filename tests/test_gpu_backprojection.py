@@ -172,3 +172,25 @@ def test_linearity_full_grid(bsar, ctx):
         out[k + "m"], _ = ds.backproject(t.gx, t.gy, t.gz, bsar.MIXED)
     assert peak_err(out["12"], a * out["1"] + b * out["2"]) <= 1e-12
     assert db(peak_err(out["12m"], a * out["1m"] + b * out["2m"])) <= MIXED_DB
+
+
+@pytest.mark.parametrize("grid", [(1, 19, 11), (3, 9, 17), (2, 8, 8), (5, 17, 1)])
+def test_voxel_bands_cover_ragged_grids(bsar, ctx, oracle, grid):
+    """The kernels take their voxels in bands of 8 rows x 8 planes; grids with
+    several bands and ragged last bands in y and z (and a single x) give every
+    voxel of the image once: FP64 against the oracle on all voxels, mixed
+    within -80 dB."""
+    t = Tiny(n=8, S=256, grid=grid)
+    t.gx = t.gx[0] + 0.01 * np.arange(grid[0])
+    t.gy = t.gy[0] + 0.01 * np.arange(grid[1])
+    t.gz = t.gz[0] + 0.02 * np.arange(grid[2])
+    root = t.random_root()
+    ds = ctx.dataset(t.params(bsar), bsar.Dem(*t.dem), t.tx, t.rx)
+    ds.load_rootdata(root)
+    img, _ = ds.backproject(t.gx, t.gy, t.gz, bsar.FP64)
+    mixed, _ = ds.backproject(t.gx, t.gy, t.gz, bsar.MIXED)
+    ds.close()
+    want = oracle.backproject(t.bp(), t.dem, t.tx, t.rx, oracle.upsample(root), t.gx, t.gy, t.gz)
+    assert np.abs(want).min() > 0
+    assert peak_err(img, want) <= FP64_TOL
+    assert db(peak_err(mixed, want)) <= MIXED_DB
