@@ -741,7 +741,7 @@ int bsar_backproject_device(bsar_ctx* c, const bsar_dataset* ds, const double* g
     int st = check_axes(c, gx, nx, gy, ny, gz, nz);
     if (st) return st;
     if (ds->device != c->device) return set_error(c, BSAR_ERR_INVALID, "dataset lives on another device");
-    if (!ds->rc) return set_error(c, BSAR_ERR_STATE, "no range-compressed data loaded");
+    if (!rcomp_ready(ds)) return set_error(c, BSAR_ERR_STATE, "no range-compressed data loaded");
     if (!d_out && (int64_t)nx * ny * nz > 0) return set_error(c, BSAR_ERR_INVALID, "d_out is null");
     BSAR_HIP(c, hipSetDevice(c->device));
     if (rep) std::memset(rep, 0, sizeof *rep);

@@ -61,6 +61,10 @@ struct bsar_dataset {
 
 extern "C" void bsar_r1tab_free(bsar_dataset* ds);
 
+// Upsampled range data fully loaded (both copies, at least one pulse): a load
+// that failed part-way leaves the dataset without, so nothing images it.
+inline bool rcomp_ready(const bsar_dataset* ds) { return ds->rc && ds->rc32 && ds->ndata > 0 && ds->S4 > 0; }
+
 #define BSAR_CHECK_HIP(ctx, call)                                     \
     do {                                                              \
         hipError_t e_ = (call);                                       \

@@ -679,7 +679,7 @@ int bsar_server_submit(bsar_server* s, const bsar_job_desc* d, int64_t* id)
     const bsar_dataset* ds = d->ds;
     if (!ds) return bad(BSAR_ERR_INVALID, "job has no dataset");
     if (ds->device != s->device) return bad(BSAR_ERR_INVALID, "dataset lives on another device");
-    if (!ds->rc) return bad(BSAR_ERR_STATE, "no range-compressed data loaded");
+    if (!rcomp_ready(ds)) return bad(BSAR_ERR_STATE, "no range-compressed data loaded");
     if (d->precision != BSAR_FP64 && d->precision != BSAR_MIXED) return bad(BSAR_ERR_INVALID, "unknown precision");
     const int K = d->pulses == 0 ? ds->n : d->pulses;
     if (K < 1 || K > ds->n) return bad(BSAR_ERR_INVALID, "pulses must be in [1, dataset pulses]");
