@@ -25,13 +25,21 @@ inline size_t fft_filter_tmp_elems(size_t S, size_t ncols, size_t elem_bytes)
 }
 hipError_t fft_filter(double2* data, double2* tmp, const double2* H, int S, int64_t ncols, hipStream_t st);
 hipError_t fft_filter(float2* data, float2* tmp, const float2* H, int S, int64_t ncols, hipStream_t st);
-// Same with the twiddle tables supplied: tw of length S / 2 and twn of length
-// S / 4 (both entries of at least one), as fft_twiddles fills them; no
-// allocation on any path.
-hipError_t fft_filter_tw(double2* data, double2* tmp, const double2* H, int S, int64_t ncols, hipStream_t st,
-                         const double2* tw, const double2* twn);
-hipError_t fft_filter_tw(float2* data, float2* tmp, const float2* H, int S, int64_t ncols, hipStream_t st,
-                         const float2* tw, const float2* twn);
+// fft_filter in two steps on one stream: fft_filter_prepare writes the
+// reordered spectrum into tmp (nothing on the global path) and builds the
+// cached twiddle tables on first use; fft_filter_apply then filters the
+// columns with tmp as prepared (the same tmp, S and ncols).  tw / twn: the
+// twiddle tables of S and S / 2 (lengths S / 2 and S / 4, at least one
+// entry each) as fft_twiddles fills them, so that no path allocates; null
+// takes the per-process tables, built on first use.
+hipError_t fft_filter_prepare(const double2* H, double2* tmp, int S, int64_t ncols, hipStream_t st,
+                              const double2* tw = nullptr, const double2* twn = nullptr);
+hipError_t fft_filter_prepare(const float2* H, float2* tmp, int S, int64_t ncols, hipStream_t st,
+                              const float2* tw = nullptr, const float2* twn = nullptr);
+hipError_t fft_filter_apply(double2* data, double2* tmp, const double2* H, int S, int64_t ncols, hipStream_t st,
+                            const double2* tw = nullptr, const double2* twn = nullptr);
+hipError_t fft_filter_apply(float2* data, float2* tmp, const float2* H, int S, int64_t ncols, hipStream_t st,
+                            const float2* tw = nullptr, const float2* twn = nullptr);
 // tw[k] = exp(-2 pi i k / S), k < max(S / 2, 1) -- the cached tables' contents.
 hipError_t fft_twiddles(double2* tw, int S, hipStream_t st);
 hipError_t fft_twiddles(float2* tw, int S, hipStream_t st);

@@ -663,6 +663,43 @@ static hipError_t fft_global(typename C2<T>::type* a, typename C2<T>::type* tmp,
 template <typename T>
 static size_t lds_bytes(int S) { return (size_t)(S + S / 32 + 1) * sizeof(typename C2<T>::type); }
 
+// The dynamic-LDS limit of a kernel, raised once per (kernel, device) to the
+// largest size asked so far, and the device's CU count, read once per device:
+// both are fixed for the life of the process, and the runtime queries cost
+// microseconds of host time on every launch otherwise.
+static hipError_t ensure_dyn_lds(const void* fn, size_t lb)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, size_t> limits;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& have = limits[{fn, dev}];
+    if (lb <= have) return hipSuccess;
+    if ((e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb)) != hipSuccess) return e;
+    have = lb;
+    return hipSuccess;
+}
+
+static hipError_t device_cus(int* cus)
+{
+    static std::mutex mu;
+    static std::map<int, int> counts;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = counts.find(dev);
+    if (it == counts.end()) {
+        int n = 0;
+        if ((e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+        it = counts.emplace(dev, std::max(n, 1)).first;
+    }
+    *cus = it->second;
+    return hipSuccess;
+}
+
 template <typename T, int KP, int Kb>
 static hipError_t launch_lds_filter(typename C2<T>::type* data, const typename C2<T>::type* Hbr,
                                     const typename C2<T>::type* tw, int logS, int64_t ncols, hipStream_t st,
@@ -671,8 +708,7 @@ static hipError_t launch_lds_filter(typename C2<T>::type* data, const typename C
     const int S = 1 << logS;
     if ((S >> KP) > fft_max_threads<T>()) return hipErrorInvalidValue;
     const size_t lb = lds_bytes<T>(S);
-    hipError_t e = hipFuncSetAttribute((const void*)k_fft_filter_lds<T, KP, Kb>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+    hipError_t e = ensure_dyn_lds((const void*)k_fft_filter_lds<T, KP, Kb>, lb);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_fft_filter_lds<T, KP, Kb>), dim3((unsigned)ncols), dim3(S >> KP), lb, st, data, data, Hbr,
                        tw, logS, halves);
@@ -686,13 +722,11 @@ static hipError_t launch_filter_pc(typename C2<T>::type* data, const typename C2
 {
     using V = typename C2<T>::type;
     const size_t lb = lds_bytes<T>((1 << LS) / NH) + 192 * sizeof(V);
-    hipError_t e = hipFuncSetAttribute((const void*)k_fft_filter_pc<T, LS, KP, Kb, R, NH, PF>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+    hipError_t e = ensure_dyn_lds((const void*)k_fft_filter_pc<T, LS, KP, Kb, R, NH, PF>, lb);
     if (e != hipSuccess) return e;
-    int dev = 0, cus = 0;
-    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    const unsigned grid = (unsigned)std::min<int64_t>(ncols, std::max(cus, 1));
+    int cus = 1;
+    if ((e = device_cus(&cus)) != hipSuccess) return e;
+    const unsigned grid = (unsigned)std::min<int64_t>(ncols, cus);
     hipLaunchKernelGGL((k_fft_filter_pc<T, LS, KP, Kb, R, NH, PF>), dim3(grid), dim3(1 << (LS - KP)), lb, st, data,
                        data, Hbr, tw, ncols);
     return hipGetLastError();
@@ -726,11 +760,14 @@ static hipError_t twiddle_table(int S, hipStream_t st, const typename C2<T>::typ
 }
 
 // tw_given / twn_given: twiddle tables of S and S / 2 the caller built
-// (fft_twiddles); null takes the cached per-process tables.
+// (fft_twiddles); null takes the cached per-process tables.  stage: kFilterAll
+// runs everything; kFilterPrepare only the spectrum reordering into tmp (and
+// the first-use twiddle tables); kFilterApply the rest, with tmp as prepared.
+enum { kFilterAll = 0, kFilterPrepare = 1, kFilterApply = 2 };
 template <typename T>
 hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, const typename C2<T>::type* H, int S,
                         int64_t ncols, hipStream_t st, const typename C2<T>::type* tw_given = nullptr,
-                        const typename C2<T>::type* twn_given = nullptr)
+                        const typename C2<T>::type* twn_given = nullptr, int stage = kFilterAll)
 {
     using V = typename C2<T>::type;
     if (S < 2 || (S & (S - 1)) || ncols <= 0) return hipErrorInvalidValue;
@@ -763,8 +800,10 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
         const V* tw = tw_given;
         hipError_t e = tw ? hipSuccess : twiddle_table<T>(S, st, &tw);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_bitrev_copy<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (stage != kFilterApply) {
+            hipLaunchKernelGGL(k_bitrev_copy<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS);
+            if ((e = hipGetLastError()) != hipSuccess || stage == kFilterPrepare) return e;
+        }
         if (pc) {
             if constexpr (std::is_same<T, double>::value)
                 return launch_filter_pc<T, 14, 5, 3, 3, 2, false>(data, tmp, tw, ncols, st);
@@ -781,13 +820,17 @@ hipError_t fft_filter_t(typename C2<T>::type* data, typename C2<T>::type* tmp, c
         hipError_t e = tw ? hipSuccess : twiddle_table<T>(S, st, &tw);
         if (e == hipSuccess && !twn) e = twiddle_table<T>(n, st, &twn);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_split_spectrum<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS - 1);
+        if (stage != kFilterApply) {
+            hipLaunchKernelGGL(k_split_spectrum<T>, dim3(nblk(S, 256)), dim3(256), 0, st, H, tmp, logS - 1);
+            if ((e = hipGetLastError()) != hipSuccess || stage == kFilterPrepare) return e;
+        }
         hipLaunchKernelGGL(k_split_dif<T>, dim3(nblk(ncols * n, 256)), dim3(256), 0, st, data, tw, n, ncols);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = lds(logS - 1, tmp, twn, 2 * ncols, 1)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_split_dit<T>, dim3(nblk(ncols * n, 256)), dim3(256), 0, st, data, tw, n, ncols);
         return hipGetLastError();
     }
+    if (stage == kFilterPrepare) return hipSuccess;   // the global path reads H as it is
     V* res = nullptr;
     hipError_t e = fft_global<T>(data, tmp, S, ncols, -1, st, &res);
     if (e != hipSuccess) return e;
@@ -817,18 +860,28 @@ hipError_t fft_filter(float2* data, float2* tmp, const float2* H, int S, int64_t
     return fft_filter_t<float>(data, tmp, H, S, ncols, st);
 }
 
-hipError_t fft_filter_tw(double2* data, double2* tmp, const double2* H, int S, int64_t ncols, hipStream_t st,
-                         const double2* tw, const double2* twn)
+hipError_t fft_filter_prepare(const double2* H, double2* tmp, int S, int64_t ncols, hipStream_t st,
+                              const double2* tw, const double2* twn)
 {
-    if (!tw || !twn) return hipErrorInvalidValue;
-    return fft_filter_t<double>(data, tmp, H, S, ncols, st, tw, twn);
+    return fft_filter_t<double>(nullptr, tmp, H, S, ncols, st, tw, twn, kFilterPrepare);
 }
 
-hipError_t fft_filter_tw(float2* data, float2* tmp, const float2* H, int S, int64_t ncols, hipStream_t st,
-                         const float2* tw, const float2* twn)
+hipError_t fft_filter_prepare(const float2* H, float2* tmp, int S, int64_t ncols, hipStream_t st,
+                              const float2* tw, const float2* twn)
 {
-    if (!tw || !twn) return hipErrorInvalidValue;
-    return fft_filter_t<float>(data, tmp, H, S, ncols, st, tw, twn);
+    return fft_filter_t<float>(nullptr, tmp, H, S, ncols, st, tw, twn, kFilterPrepare);
+}
+
+hipError_t fft_filter_apply(double2* data, double2* tmp, const double2* H, int S, int64_t ncols, hipStream_t st,
+                            const double2* tw, const double2* twn)
+{
+    return fft_filter_t<double>(data, tmp, H, S, ncols, st, tw, twn, kFilterApply);
+}
+
+hipError_t fft_filter_apply(float2* data, float2* tmp, const float2* H, int S, int64_t ncols, hipStream_t st,
+                            const float2* tw, const float2* twn)
+{
+    return fft_filter_t<float>(data, tmp, H, S, ncols, st, tw, twn, kFilterApply);
 }
 
 template <typename T>

