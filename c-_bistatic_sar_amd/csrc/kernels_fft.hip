@@ -485,8 +485,20 @@ __global__ __launch_bounds__(1 << (LS - KP), 1) void k_fft_filter_pc(const typen
         }
         butterflies<T, KP, false, true, false, true, TwLds<V>>(v, t, TOP, LS, ltw);
         const T inv = (T)1 / (T)S;
+        if constexpr (PF) {
+            // streaming (non-temporal) stores: the result is not read again
+            // here, and keeping it out of L2 measured 2 % faster for complex64
+            // columns (no change for complex128)
+            typedef T NV __attribute__((ext_vector_type(2)));
 #pragma unroll
-        for (int i = 0; i < P; ++i) out[col * S + t + i * NT] = V{v[i].x * inv, v[i].y * inv};
+            for (int i = 0; i < P; ++i) {
+                const NV w = {v[i].x * inv, v[i].y * inv};
+                __builtin_nontemporal_store(w, reinterpret_cast<NV*>(&out[col * S + t + i * NT]));
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < P; ++i) out[col * S + t + i * NT] = V{v[i].x * inv, v[i].y * inv};
+        }
         if constexpr (PF) {
 #pragma unroll
             for (int i = 0; i < P; ++i) v[i] = vn[i];
