@@ -178,7 +178,9 @@ BSAR_API int bsar_fft(bsar_ctx* ctx, double* data, int S, int N, int inverse);
 /* Same filter on a caller-owned device buffer, in place: d_sig is [N][S]
  * complex128 (BSAR_FP64) or complex64 (BSAR_MIXED), S a power of two; ref is
  * the host reference (complex128, length S).  rep (optional) receives the
- * device time of the batched FFT -> multiply -> IFFT. */
+ * device time of the batched FFT -> multiply -> IFFT over the columns; the
+ * reference's own spectrum (its FFT, conjugation and reordering) is made
+ * before that interval and is not included. */
 BSAR_API int bsar_range_compress_device(bsar_ctx* ctx, void* d_sig, int S, int N, const double* ref, int conj,
                                int precision, bsar_report* rep);
 
